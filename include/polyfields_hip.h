@@ -1,0 +1,155 @@
+/*
+ * polyfields_hip.h -- C ABI of the MI355X backend for PolyFields.jl's SCFT hot path.
+ *
+ * One FieldSystem lives on one GPU; the library owns its device grids, the host owns
+ * every buffer it passes in.  Grids cross the boundary as float64 arrays of n = nx*ny*nz
+ * entries in the reference's memory order (Julia Array{Float64,3}: x fastest), so a
+ * Julia `ccall` passes `pointer(A)` without a transpose.  Every function returns 0 on
+ * success and a negative code on failure; `pf_last_error()` returns the message (thread
+ * local).  Codes: -1 bad argument, -2 invalid model or state, -3 HIP runtime error,
+ * -4 singular matrix, -5 divergence.  A system and the updaters bound to it are not
+ * re-entrant: one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to PolyFields.jl/src):
+ *   pf_system_create ........ FieldSystem(dims, cell; ensemble, compress, mde)  system/system.jl:51-74
+ *                             with UnitCell(dim, crystal, params...)             system/cell.jl:31-54
+ *   pf_system_add_monomer ... add_monomer!(sys, Monomer(id, vol, charge))       system/system.jl:113-128
+ *   pf_system_add_multiblock  add_species!(sys, Multiblock(...), phi)           system/system.jl:137-161,
+ *                                                                               species/multiblock.jl:32-77
+ *   pf_system_add_point ..... add_species!(sys, Point(mon), phi)                species/point.jl:19-22
+ *   pf_system_set_flory ..... set_interaction!(FloryInteraction, a, b, chi)     interactions/flory.jl:25-37
+ *   pf_system_set_edwards ... set_interaction!(EdwardsInteraction, a, u)        interactions/edwards.jl:25-32
+ *   pf_system_set_field ..... sys.fields[mid] .= w (fieldinit! dict form)       fileio.jl:25-54
+ *   pf_system_get_grid ...... sys.fields / density / residuals / potentials     system/system.jl:35-38
+ *   pf_system_get_species_density  species.density[mid]                         species/multiblock.jl:82
+ *   pf_system_get_propagator  species.q[s] / species.qc[s]                      species/multiblock.jl:80-81
+ *   pf_system_species_scalars species.Q, sys.phi_species, sys.mu_species         system/system.jl:44-45
+ *   pf_system_cell .......... cell.params / update!(cell)                       system/cell.jl:96-109
+ *   pf_validate ............. validate(sys)                                     system/system.jl:192-237
+ *   pf_density .............. density!(sys)                                     system/system.jl:266-289
+ *   pf_residuals ............ residuals!(sys)                                   system/system.jl:317-340
+ *   pf_field_error .......... field_error(sys)                                  system/system.jl:470-486
+ *   pf_free_energy .......... free_energy(sys)                                  system/system.jl:402-433
+ *   pf_free_energy_bulk ..... free_energy_bulk(sys)                             system/system.jl:440-458
+ *   pf_scfstress ............ scfstress(sys)                                    system/system.jl:384-395
+ *   pf_updater_anderson ..... AndersonUpdater(; nhist, iter_euler, lam_euler)   updaters/anderson.jl:36-48
+ *   pf_updater_euler ........ EulerUpdater(; method, lam)                       updaters/euler.jl:20-28
+ *   pf_updater_momentum ..... MomentumUpdater(; method, lam, gamma, beta1, beta2) updaters/momentum.jl:30-41
+ *   pf_updater_setup/step ... setup!(u, sys) / step!(u)                         updaters/base.jl:7-9
+ *   pf_domain_create ........ DomainUpdater(; nper, nskip, lam, pmax, rlxn)     updaters/domain.jl:20-24
+ *   pf_domain_setup/step .... setup!(d, sys) / step!(d; nsteps, tol)            updaters/domain.jl:32-92
+ *   pf_domain_stress ........ domain.stress / stress_error(d)                   updaters/domain.jl:30,44-48
+ *   pf_scft ................. scft!(sys, updater; domain, opts) -> SCFTResults  simulate.jl:60-148
+ *   pf_cell_geometry ........ make_basis / basis! / dbasis! / ddbasis!          system/basis.jl:11-116,
+ *                                                                               system/cell.jl:113-258
+ *   pf_chain_grid ........... chain_grid(N_block, Ns)                           species/base.jl:27-56
+ */
+#ifndef POLYFIELDS_HIP_H
+#define POLYFIELDS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_PRECISION_FP64 0 /* every grid and transform in float64 ("plain") */
+#define PF_PRECISION_FP32 1 /* float32 propagator and transforms, float64 fields and sums */
+
+#define PF_CANONICAL 0
+#define PF_GRAND 1
+
+#define PF_GRID_FIELD 0
+#define PF_GRID_DENSITY 1
+#define PF_GRID_RESIDUAL 2
+#define PF_GRID_POTENTIAL 3
+
+typedef struct pf_system pf_system;
+typedef struct pf_updater pf_updater;
+typedef struct pf_domain pf_domain;
+
+typedef struct {
+    int nsteps;    /* SCFTOptions.nsteps (default 100)  */
+    int nsout;     /* print progress every nsout steps (0: quiet) */
+    int nsave;     /* accepted, unused: field files are the caller's business */
+    double ftol;   /* default 1e-5 */
+    double stol;   /* default 1e-4 */
+} pf_scft_options;
+
+typedef struct {
+    double time;
+    int nsteps;
+    int converged;
+    double ferr;
+    double serr;
+    double fhelm;
+} pf_scft_results;
+
+const char* pf_last_error(void);
+int pf_version(void);
+int pf_device_count(int* count);
+
+/* Host-only helpers (no GPU needed). R9/G9 are row-major 3x3 (column j = vector j);
+   dGG has 9*nparams entries indexed [(i*3+j)*nparams + k]. */
+int pf_cell_geometry(int dim, const char* crystal, const double* params, int nparams,
+                     double* R9, double* G9, double* volume, double* dGG);
+int pf_chain_grid(int nblk, const double* Nref_block, int Ns, int* begin /* nblk+1, 1-based */,
+                  double* ds_block, double* Nref);
+
+/* System */
+int pf_system_create(pf_system** out, int nx, int ny, int nz, int dim, const char* crystal,
+                     const double* params, int nparams, int ensemble, double inv_zeta,
+                     int precision, int device);
+int pf_system_destroy(pf_system* sys);
+int pf_system_add_monomer(pf_system* sys, int id, double vol, double charge);
+int pf_system_add_multiblock(pf_system* sys, int nblk, const int* block_mids, const int* N_block,
+                             const double* b_block, int Ns, double phi_or_mu, int* species_index,
+                             int* Ns_used);
+int pf_system_add_point(pf_system* sys, int mid, double phi_or_mu, int* species_index);
+int pf_system_set_flory(pf_system* sys, int mid_a, int mid_b, double chi);
+int pf_system_set_edwards(pf_system* sys, int mid, double u);
+int pf_system_set_field(pf_system* sys, int mid, const double* src);
+int pf_system_get_grid(pf_system* sys, int which, int mid, double* dst);
+int pf_system_get_species_density(pf_system* sys, int species, int mid, double* dst);
+int pf_system_get_propagator(pf_system* sys, int species, int s, int backward, double* dst);
+int pf_system_species_scalars(pf_system* sys, int species, double* Q, double* phi, double* mu);
+/* set != 0: write params (length nparams) and update!(cell); else read them. */
+int pf_system_cell(pf_system* sys, double* params, int nparams, int set);
+int pf_system_synchronize(pf_system* sys);
+
+int pf_validate(pf_system* sys);
+int pf_density(pf_system* sys);
+int pf_residuals(pf_system* sys);
+int pf_field_error(pf_system* sys, double* ferr);
+int pf_free_energy(pf_system* sys, double* fhelm);
+int pf_free_energy_bulk(pf_system* sys, double* fhelm);
+int pf_scfstress(pf_system* sys, double* stress, int nparams);
+
+/* Field updaters.  method: Euler 0 PEC, 1 PECE; Momentum 0 MOM, 1 RMS, 2 ADAM, 3 NAG. */
+int pf_updater_anderson(pf_updater** out, int nhist, int iter_euler, double lam_euler);
+int pf_updater_euler(pf_updater** out, int method, double lam);
+int pf_updater_momentum(pf_updater** out, int method, double lam, double gamma, double beta1,
+                        double beta2);
+int pf_updater_setup(pf_updater* u, pf_system* sys);
+int pf_updater_step(pf_updater* u);
+int pf_updater_destroy(pf_updater* u);
+
+/* Variable cell */
+int pf_domain_create(pf_domain** out, int nper, int nskip, double lam, double pmax, double rlxn);
+int pf_domain_setup(pf_domain* d, pf_system* sys);
+int pf_domain_step(pf_domain* d, int nsteps, double tol);
+int pf_domain_stress(pf_domain* d, int refresh, double* stress, int nparams, double* serr);
+int pf_domain_destroy(pf_domain* d);
+
+/* Driver.  domain may be NULL (fixed cell). */
+int pf_scft(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
+            pf_scft_results* res);
+
+/* Measurement: per-launch HIP-event timing of the propagator kernels on the system's
+   stream.  kind: 0 xy-slab pass, 1 z-pencil pass, 2 whole-plane (1-D/2-D) sweep. */
+int pf_system_timing(pf_system* sys, int on);
+int pf_system_timing_read(pf_system* sys, int kind, double* total_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POLYFIELDS_HIP_H */

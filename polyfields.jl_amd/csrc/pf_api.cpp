@@ -1,0 +1,387 @@
+// extern "C" boundary: argument checks, exception capture, handle management.
+#include <cstring>
+
+#include "../../include/polyfields_hip.h"
+#include "pf_host.h"
+
+struct pf_system {
+    pf::System* s;
+};
+struct pf_updater {
+    pf::Updater* u;
+};
+struct pf_domain {
+    pf::Domain* d;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const pf::Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -2;
+    } catch (...) {
+        g_err = "unknown error";
+        return -2;
+    }
+}
+
+void need(bool ok, const char* what) {
+    if (!ok) throw pf::Error(-1, std::string("invalid argument: ") + what);
+}
+
+pf::System& S(pf_system* p) {
+    need(p && p->s, "system handle");
+    return *p->s;
+}
+}  // namespace
+
+extern "C" {
+
+const char* pf_last_error(void) { return g_err.c_str(); }
+
+int pf_version(void) { return 1; }
+
+int pf_device_count(int* count) {
+    return guard([&] {
+        need(count, "count");
+        int c = 0;
+        PF_HIP(hipGetDeviceCount(&c));
+        *count = c;
+    });
+}
+
+int pf_cell_geometry(int dim, const char* crystal, const double* params, int nparams,
+                     double* R9, double* G9, double* volume, double* dGG) {
+    return guard([&] {
+        need(crystal && params && nparams > 0 && nparams <= PF_MAXPAR, "cell parameters");
+        pf::Cell c;
+        c.dim = dim;
+        c.crystal = crystal;
+        c.params.assign(params, params + nparams);
+        c.update();
+        if (R9) std::memcpy(R9, c.R, sizeof(c.R));
+        if (G9) std::memcpy(G9, c.G, sizeof(c.G));
+        if (volume) *volume = c.volume;
+        if (dGG) std::memcpy(dGG, c.dGG.data(), sizeof(double) * c.dGG.size());
+    });
+}
+
+int pf_chain_grid(int nblk, const double* Nref_block, int Ns, int* begin, double* ds_block,
+                  double* Nref) {
+    return guard([&] {
+        need(nblk > 0 && Nref_block && Ns > 0 && begin && ds_block && Nref, "chain grid");
+        std::vector<double> nb(Nref_block, Nref_block + nblk);
+        std::vector<int> bg;
+        std::vector<double> ds;
+        pf::chain_grid(nb, Ns, Nref, bg, ds);
+        for (int i = 0; i <= nblk; ++i) begin[i] = bg[i];
+        for (int i = 0; i < nblk; ++i) ds_block[i] = ds[i];
+    });
+}
+
+int pf_system_create(pf_system** out, int nx, int ny, int nz, int dim, const char* crystal,
+                     const double* params, int nparams, int ensemble, double inv_zeta,
+                     int precision, int device) {
+    return guard([&] {
+        need(out && crystal && params && nparams > 0 && nparams <= PF_MAXPAR, "system create");
+        need(ensemble == PF_CANONICAL || ensemble == PF_GRAND, "ensemble");
+        std::vector<double> p(params, params + nparams);
+        auto* h = new pf_system{nullptr};
+        try {
+            h->s = new pf::System(nx, ny, nz, dim, crystal, p, precision,
+                                  ensemble == PF_CANONICAL, inv_zeta, device);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int pf_system_destroy(pf_system* sys) {
+    return guard([&] {
+        if (!sys) return;
+        delete sys->s;
+        delete sys;
+    });
+}
+
+int pf_system_add_monomer(pf_system* sys, int id, double vol, double charge) {
+    return guard([&] { S(sys).add_monomer(id, vol, charge); });
+}
+
+int pf_system_add_multiblock(pf_system* sys, int nblk, const int* block_mids, const int* N_block,
+                             const double* b_block, int Ns, double phi_or_mu, int* species_index,
+                             int* Ns_used) {
+    return guard([&] {
+        need(nblk > 0 && block_mids && N_block && b_block, "multiblock arrays");
+        auto& s = S(sys);
+        int i = s.add_chain(std::vector<int>(block_mids, block_mids + nblk),
+                            std::vector<int>(N_block, N_block + nblk),
+                            std::vector<double>(b_block, b_block + nblk), Ns, phi_or_mu);
+        if (species_index) *species_index = i;
+        if (Ns_used) *Ns_used = s.species[i].Ns;
+    });
+}
+
+int pf_system_add_point(pf_system* sys, int mid, double phi_or_mu, int* species_index) {
+    return guard([&] {
+        int i = S(sys).add_point(mid, phi_or_mu);
+        if (species_index) *species_index = i;
+    });
+}
+
+int pf_system_set_flory(pf_system* sys, int a, int b, double chi) {
+    return guard([&] { S(sys).set_flory(a, b, chi); });
+}
+
+int pf_system_set_edwards(pf_system* sys, int mid, double u) {
+    return guard([&] { S(sys).set_edwards(mid, u); });
+}
+
+int pf_system_set_field(pf_system* sys, int mid, const double* src) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(src, "field source");
+        const int m = s.slot_of(mid);
+        need(m >= 0, "monomer id");
+        PF_HIP(hipMemcpyAsync(s.w + (size_t)m * s.n, src, sizeof(double) * s.n,
+                              hipMemcpyHostToDevice, s.stream));
+        s.sync();
+        s.fields_set = true;
+    });
+}
+
+int pf_system_get_grid(pf_system* sys, int which, int mid, double* dst) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(dst, "destination");
+        const int m = s.slot_of(mid);
+        need(m >= 0, "monomer id");
+        const double* base = which == PF_GRID_FIELD      ? s.w
+                             : which == PF_GRID_DENSITY  ? s.rho
+                             : which == PF_GRID_RESIDUAL ? s.res
+                             : which == PF_GRID_POTENTIAL ? s.pot
+                                                          : nullptr;
+        need(base, "grid kind");
+        PF_HIP(hipMemcpyAsync(dst, base + (size_t)m * s.n, sizeof(double) * s.n,
+                              hipMemcpyDeviceToHost, s.stream));
+        s.sync();
+    });
+}
+
+int pf_system_get_species_density(pf_system* sys, int species, int mid, double* dst) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(dst && species >= 0 && species < (int)s.species.size(), "species");
+        const auto& sp = s.species[species];
+        const int m = s.slot_of(mid);
+        auto it = std::find(sp.slots.begin(), sp.slots.end(), m);
+        need(m >= 0 && it != sp.slots.end(), "monomer of species");
+        PF_HIP(hipMemcpyAsync(dst, sp.rho_sp + (size_t)(it - sp.slots.begin()) * s.n,
+                              sizeof(double) * s.n, hipMemcpyDeviceToHost, s.stream));
+        s.sync();
+    });
+}
+
+int pf_system_get_propagator(pf_system* sys, int species, int sl, int backward, double* dst) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(dst && species >= 0 && species < (int)s.species.size(), "species");
+        s.eng->get_propagator(s, species, sl, backward, dst);
+    });
+}
+
+int pf_system_species_scalars(pf_system* sys, int species, double* Q, double* phi, double* mu) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(species >= 0 && species < (int)s.species.size(), "species");
+        s.fetch_scalars();
+        if (Q) *Q = s.Q[species];
+        if (phi) *phi = s.phi[species];
+        if (mu) *mu = s.mu[species];
+    });
+}
+
+int pf_system_cell(pf_system* sys, double* params, int nparams, int set) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(params && nparams == (int)s.cell.params.size(), "cell parameter count");
+        if (set) {
+            s.cell.params.assign(params, params + nparams);
+            s.cell.update();
+        } else {
+            for (int k = 0; k < nparams; ++k) params[k] = s.cell.params[k];
+        }
+    });
+}
+
+int pf_system_synchronize(pf_system* sys) {
+    return guard([&] { S(sys).sync(); });
+}
+
+int pf_validate(pf_system* sys) {
+    return guard([&] { S(sys).validate(); });
+}
+
+int pf_density(pf_system* sys) {
+    return guard([&] { S(sys).density(); });
+}
+
+int pf_residuals(pf_system* sys) {
+    return guard([&] { S(sys).residuals(); });
+}
+
+int pf_field_error(pf_system* sys, double* v) {
+    return guard([&] {
+        need(v, "output");
+        *v = S(sys).field_error();
+    });
+}
+
+int pf_free_energy(pf_system* sys, double* v) {
+    return guard([&] {
+        need(v, "output");
+        *v = S(sys).free_energy();
+    });
+}
+
+int pf_free_energy_bulk(pf_system* sys, double* v) {
+    return guard([&] {
+        need(v, "output");
+        *v = S(sys).free_energy_bulk();
+    });
+}
+
+int pf_scfstress(pf_system* sys, double* stress, int nparams) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(stress && nparams == (int)s.cell.params.size(), "stress output");
+        auto v = s.scfstress();
+        for (int k = 0; k < nparams; ++k) stress[k] = v[k];
+    });
+}
+
+int pf_updater_anderson(pf_updater** out, int nhist, int iter_euler, double lam_euler) {
+    return guard([&] {
+        need(out, "output");
+        *out = new pf_updater{new pf::Anderson(nhist, iter_euler, lam_euler)};
+    });
+}
+
+int pf_updater_euler(pf_updater** out, int method, double lam) {
+    return guard([&] {
+        need(out && (method == 0 || method == 1), "Euler method (0 PEC, 1 PECE)");
+        *out = new pf_updater{new pf::Euler(method, lam)};
+    });
+}
+
+int pf_updater_momentum(pf_updater** out, int method, double lam, double gamma, double beta1,
+                        double beta2) {
+    return guard([&] {
+        need(out && method >= 0 && method <= 3, "Momentum method (0 MOM, 1 RMS, 2 ADAM, 3 NAG)");
+        need(gamma > 0 && gamma < 1 && beta1 > 0 && beta1 < 1 && beta2 > 0 && beta2 < 1,
+             "momentum coefficients in (0, 1)");
+        *out = new pf_updater{new pf::Momentum(method, lam, gamma, beta1, beta2)};
+    });
+}
+
+int pf_updater_setup(pf_updater* u, pf_system* sys) {
+    return guard([&] {
+        need(u && u->u, "updater");
+        u->u->setup(&S(sys));
+    });
+}
+
+int pf_updater_step(pf_updater* u) {
+    return guard([&] {
+        need(u && u->u && u->u->sys, "updater bound to a system");
+        u->u->step();
+    });
+}
+
+int pf_updater_destroy(pf_updater* u) {
+    return guard([&] {
+        if (!u) return;
+        delete u->u;
+        delete u;
+    });
+}
+
+int pf_domain_create(pf_domain** out, int nper, int nskip, double lam, double pmax, double rlxn) {
+    return guard([&] {
+        need(out && nper >= 0 && nskip > 0, "domain parameters");
+        *out = new pf_domain{new pf::Domain(nper, nskip, lam, pmax, rlxn)};
+    });
+}
+
+int pf_domain_setup(pf_domain* d, pf_system* sys) {
+    return guard([&] {
+        need(d && d->d, "domain");
+        d->d->setup(&S(sys));
+    });
+}
+
+int pf_domain_step(pf_domain* d, int nsteps, double tol) {
+    return guard([&] {
+        need(d && d->d && d->d->sys, "domain bound to a system");
+        d->d->step(nsteps, tol);
+    });
+}
+
+int pf_domain_stress(pf_domain* d, int refresh, double* stress, int nparams, double* serr) {
+    return guard([&] {
+        need(d && d->d && d->d->sys, "domain bound to a system");
+        if (refresh) d->d->scfstress_update();
+        if (stress) {
+            need(nparams == (int)d->d->stress.size(), "stress length");
+            for (int k = 0; k < nparams; ++k) stress[k] = d->d->stress[k];
+        }
+        if (serr) *serr = d->d->stress_error();
+    });
+}
+
+int pf_domain_destroy(pf_domain* d) {
+    return guard([&] {
+        if (!d) return;
+        delete d->d;
+        delete d;
+    });
+}
+
+int pf_scft(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
+            pf_scft_results* res) {
+    return guard([&] {
+        need(u && u->u && opts && res, "scft arguments");
+        pf::ScftOptions o{opts->nsteps, opts->nsout, opts->nsave, opts->ftol, opts->stol};
+        pf::ScftResults r = pf::scft(S(sys), *u->u, d ? d->d : nullptr, o);
+        res->time = r.time;
+        res->nsteps = r.nsteps;
+        res->converged = r.converged;
+        res->ferr = r.ferr;
+        res->serr = r.serr;
+        res->fhelm = r.fhelm;
+    });
+}
+
+int pf_system_timing(pf_system* sys, int on) {
+    return guard([&] { S(sys).eng->timing(on); });
+}
+
+int pf_system_timing_read(pf_system* sys, int kind, double* total_ms, int* launches) {
+    return guard([&] {
+        need(total_ms, "output");
+        *total_ms = S(sys).eng->timing_ms(kind, launches);
+    });
+}
+
+}  // extern "C"

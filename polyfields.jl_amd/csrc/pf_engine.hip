@@ -1,0 +1,415 @@
+// Propagator engine: owns the q / q+ stacks, LW tables, FFT plans and twiddles of one
+// system, and sequences the sweep kernels for density!(sys) and the stress pass.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pf_host.h"
+
+namespace pf {
+
+static FftPlan make_plan(int n) {
+    FftPlan p{};
+    p.n = n;
+    int m = n;
+    auto push = [&](int r) {
+        if (p.npass >= PF_MAXPASS) throw Error(-2, "transform length has too many factors");
+        p.radix[p.npass++] = (int8_t)r;
+        m /= r;
+    };
+    while (m % 4 == 0) push(4);
+    while (m % 2 == 0) push(2);
+    while (m % 3 == 0) push(3);
+    if (m != 1)
+        throw Error(-2, "grid size " + std::to_string(n) +
+                            " has a prime factor other than 2 or 3 (unsupported)");
+    return p;
+}
+
+template <typename T>
+static std::vector<cpx<T>> twiddles(int n) {
+    std::vector<cpx<T>> t(std::max(n, 1));
+    for (int m = 0; m < n; ++m) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / n;
+        t[m] = {(T)(double)cosl(a), (T)(double)sinl(a)};
+    }
+    return t;
+}
+
+template <typename T>
+class Engine : public EngineBase {
+    struct SpBuf {
+        T* q = nullptr;
+        T* qc = nullptr;
+        std::vector<T*> lw;
+        int row0 = 0;              // first stress row of this species
+    };
+    System* S = nullptr;
+    std::vector<SpBuf> sb;
+    cpx<T>* tw = nullptr;
+    const cpx<T>*twx = nullptr, *twy = nullptr, *twz = nullptr;
+    FftPlan px{}, py{}, pz{};
+    std::vector<cpx<T>*> kbufs;
+    std::vector<Lane<T>> lanes;
+    Lane<T>* d_lanes = nullptr;
+    std::vector<SimpsonTask<T>> tasks;
+    SimpsonTask<T>* d_tasks = nullptr;
+    long lanes_cell_version = -1;
+    int maxNs = 0;
+    int planar = 1;
+    int ntiles = 1, tile = 1;
+    size_t nk = 0;
+    int lds_plane = 0, lds_pencil = 0;
+    // stress
+    std::vector<StressLane<T>> slanes;
+    StressLane<T>* d_slanes = nullptr;
+    long slanes_cell_version = -1;
+    int nrows = 0;
+    cpx<T>* kstress = nullptr;
+    int schunk = 0;
+    double* srows = nullptr;
+    double* spart = nullptr;
+    // timing of the propagator kernels: 0 slab, 1 pencil, 2 planar
+    bool tim = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evlog;
+    size_t evnext = 0;
+
+    void free_all() {
+        for (auto& b : sb) {
+            dev_free(b.q);
+            dev_free(b.qc);
+            for (T* p : b.lw) dev_free(p);
+        }
+        sb.clear();
+        for (auto* k : kbufs) dev_free(k);
+        kbufs.clear();
+        dev_free(d_lanes);
+        dev_free(d_tasks);
+        dev_free(d_slanes);
+        dev_free(kstress);
+        dev_free(srows);
+        dev_free(spart);
+        d_lanes = nullptr;
+        d_tasks = nullptr;
+        d_slanes = nullptr;
+        kstress = nullptr;
+        srows = spart = nullptr;
+        slanes.clear();
+    }
+
+    void upload(void* dst, const void* src, size_t bytes) {
+        PF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S->stream));
+        PF_HIP(hipStreamSynchronize(S->stream));
+    }
+
+    void build() {
+        System& s = *S;
+        free_all();
+        const int nx = s.dims[0], ny = s.dims[1], nz = s.dims[2];
+        const size_t n = s.n;
+        planar = (nz == 1);
+        nk = (size_t)(nx / 2 + 1) * ny * nz;
+        if (!tw) {
+            px = make_plan(nx / 2);
+            py = make_plan(ny);
+            pz = make_plan(nz);
+            auto tx = twiddles<T>(nx), ty = twiddles<T>(ny), tz = twiddles<T>(nz);
+            tw = (cpx<T>*)dev_alloc(sizeof(cpx<T>) * (tx.size() + ty.size() + tz.size()));
+            upload(tw, tx.data(), sizeof(cpx<T>) * tx.size());
+            upload(tw + tx.size(), ty.data(), sizeof(cpx<T>) * ty.size());
+            upload(tw + tx.size() + ty.size(), tz.data(), sizeof(cpx<T>) * tz.size());
+            twx = tw;
+            twy = tw + tx.size();
+            twz = tw + tx.size() + ty.size();
+        }
+        const int nxh = nx / 2 + 1;
+        ntiles = (nxh + 31) / 32;
+        tile = (nxh + ntiles - 1) / ntiles;
+        lds_plane = (int)(sizeof(cpx<T>) * ny * nxh);
+        lds_pencil = (int)(sizeof(cpx<T>) * nz * tile);
+        if (lds_plane > 160 * 1024)
+            throw Error(-2, "xy plane of " + std::to_string(nx) + "x" + std::to_string(ny) +
+                                " does not fit the 160 KiB LDS of one workgroup");
+        maxNs = 0;
+        lanes.clear();
+        tasks.clear();
+        sb.resize(s.species.size());
+        int row = 0;
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            const Species& sp = s.species[i];
+            if (sp.kind != 0) continue;
+            SpBuf& b = sb[i];
+            b.row0 = row;
+            row += sp.Ns + 1;
+            b.q = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
+            if (sp.nblk() > 1) b.qc = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
+            for (int k = 0; k < sp.nblk(); ++k) b.lw.push_back((T*)dev_alloc(sizeof(T) * n));
+            maxNs = std::max(maxNs, sp.Ns);
+            for (int d = 0; d < (sp.nblk() > 1 ? 2 : 1); ++d) {
+                Lane<T> L{};
+                L.nx = nx;
+                L.ny = ny;
+                L.nz = nz;
+                L.dir = d == 0 ? 1 : -1;
+                L.nsteps = sp.Ns;
+                L.nblk = sp.nblk();
+                for (int k = 0; k <= sp.nblk(); ++k) L.begin[k] = sp.begin[k];
+                for (int k = 0; k < sp.nblk(); ++k) {
+                    L.lw[k] = b.lw[k];
+                    const double bb = sp.b_block[k];
+                    L.ldc[k] = sp.ds_block[k] * (bb * bb) / 6.0;
+                }
+                L.stack = d == 0 ? b.q : b.qc;
+                if (!planar) {
+                    kbufs.push_back((cpx<T>*)dev_alloc(sizeof(cpx<T>) * nk));
+                    L.kbuf = kbufs.back();
+                }
+                L.twx = twx;
+                L.twy = twy;
+                L.twz = twz;
+                L.px = px;
+                L.py = py;
+                L.pz = pz;
+                lanes.push_back(L);
+            }
+            SimpsonTask<T> tk{};
+            tk.q = b.q;
+            tk.qc = b.qc;
+            tk.Ns = sp.Ns;
+            tk.nblk = sp.nblk();
+            for (int k = 0; k <= sp.nblk(); ++k) tk.begin[k] = sp.begin[k];
+            for (int k = 0; k < sp.nblk(); ++k) {
+                tk.ds[k] = sp.ds_block[k];
+                const int slot = sp.block_slot[k];
+                tk.blk_mon[k] = (int)(std::find(sp.slots.begin(), sp.slots.end(), slot) -
+                                      sp.slots.begin());
+            }
+            tk.Nref = sp.Nref;
+            tk.isp = (int)i;
+            tk.rho_sp = sp.rho_sp;
+            tk.nmon = (int)sp.slots.size();
+            tasks.push_back(tk);
+        }
+        nrows = row;
+        if (!lanes.empty()) d_lanes = (Lane<T>*)dev_alloc(sizeof(Lane<T>) * lanes.size());
+        if (!tasks.empty()) {
+            d_tasks = (SimpsonTask<T>*)dev_alloc(sizeof(SimpsonTask<T>) * tasks.size());
+            upload(d_tasks, tasks.data(), sizeof(SimpsonTask<T>) * tasks.size());
+        }
+        lanes_cell_version = -1;
+        slanes_cell_version = -1;
+        s.dirty_species = 0;
+    }
+
+    void refresh_lanes() {
+        if (lanes.empty() || lanes_cell_version == S->cell.version) return;
+        for (auto& L : lanes) std::memcpy(L.G, S->cell.G, sizeof(L.G));
+        upload(d_lanes, lanes.data(), sizeof(Lane<T>) * lanes.size());
+        lanes_cell_version = S->cell.version;
+    }
+
+    hipEvent_t ev() {
+        if (evnext >= evpool.size()) {
+            hipEvent_t e;
+            PF_HIP(hipEventCreate(&e));
+            evpool.push_back(e);
+        }
+        return evpool[evnext++];
+    }
+    template <typename F>
+    void timed(int cls, F&& f) {
+        if (!tim) {
+            f();
+            return;
+        }
+        hipEvent_t a = ev(), b = ev();
+        PF_HIP(hipEventRecord(a, S->stream));
+        f();
+        PF_HIP(hipEventRecord(b, S->stream));
+        evlog.push_back({cls, {a, b}});
+    }
+
+   public:
+    explicit Engine(System& s) : S(&s) {}
+    ~Engine() override {
+        free_all();
+        dev_free(tw);
+        for (auto e : evpool) (void)hipEventDestroy(e);
+    }
+
+    void timing(int on) override {
+        tim = on != 0;
+        evlog.clear();
+        evnext = 0;
+    }
+
+    double timing_ms(int which, int* count) override {
+        PF_HIP(hipStreamSynchronize(S->stream));
+        double tot = 0.0;
+        int c = 0;
+        for (auto& e : evlog)
+            if (e.first == which) {
+                float ms = 0.f;
+                PF_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+                tot += ms;
+                ++c;
+            }
+        if (count) *count = c;
+        return tot;
+    }
+
+    void density(System& s) override {
+        if (s.dirty_species || sb.size() != s.species.size()) build();
+        refresh_lanes();
+        hipStream_t st = s.stream;
+        const size_t n = s.n;
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            const Species& sp = s.species[i];
+            if (sp.kind != 0) continue;
+            for (int k = 0; k < sp.nblk(); ++k) {
+                const int slot = sp.block_slot[k];
+                const double coef = (-(sp.ds_block[k] / 2.0)) * s.mon_vol[slot];
+                launch_lw_table<T>(sb[i].lw[k], s.w + (size_t)slot * n, coef, n, st);
+            }
+            launch_fill<T>(sb[i].q, T(1), n, st);
+            if (sb[i].qc) launch_fill<T>(sb[i].qc + (size_t)sp.Ns * n, T(1), n, st);
+        }
+        const int nl = (int)lanes.size();
+        if (nl) {
+            if (planar) {
+                timed(2, [&] { launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st); });
+            } else {
+                const int nz = s.dims[2], ny = s.dims[1];
+                timed(0, [&] { launch_slab<T>(d_lanes, nl, nz, SLAB_FWD, 0, lds_plane, 256, st); });
+                for (int t = 0; t < maxNs; ++t) {
+                    timed(1, [&] {
+                        launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st);
+                    });
+                    timed(0, [&] {
+                        launch_slab<T>(d_lanes, nl, nz, SLAB_INVFWD, t, lds_plane, 256, st);
+                    });
+                }
+            }
+        }
+        const int nparts = reduce_parts(n);
+        if (!tasks.empty())
+            launch_species_Q<T>(d_tasks, (int)tasks.size(), n, s.partial, nparts, s.sc, st);
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            const Species& sp = s.species[i];
+            if (sp.kind != 1) continue;
+            const int slot = sp.slots[0];
+            launch_point_density(s.w + (size_t)slot * n, s.mon_vol[slot], n, sp.rho_sp,
+                                 s.partial, nparts, s.sc, (int)i, st);
+        }
+        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), n, s.sc, st);
+    }
+
+    void build_stress() {
+        System& s = *S;
+        const int nx = s.dims[0], ny = s.dims[1], nz = s.dims[2];
+        const size_t n = s.n;
+        const bool fused = planar && 2 * lds_plane <= 160 * 1024;
+        if (slanes.empty() || (int)slanes.size() != nrows) {
+            dev_free(d_slanes);
+            dev_free(kstress);
+            dev_free(srows);
+            dev_free(spart);
+            d_slanes = nullptr;
+            kstress = nullptr;
+            slanes.assign(nrows, StressLane<T>{});
+            if (nrows) d_slanes = (StressLane<T>*)dev_alloc(sizeof(StressLane<T>) * nrows);
+            srows = (double*)dev_alloc(sizeof(double) * std::max(nrows, 1) * PF_MAXPAR);
+            if (!fused) {
+                schunk = (int)std::max<size_t>(1, std::min<size_t>(nrows, (size_t)(256u << 20) /
+                                                                             (2 * nk * sizeof(cpx<T>))));
+                kstress = (cpx<T>*)dev_alloc(sizeof(cpx<T>) * 2 * nk * schunk);
+                spart = (double*)dev_alloc(sizeof(double) * (size_t)std::max(nrows, 1) * ny *
+                                           ntiles * PF_MAXPAR);
+            }
+        }
+        const int np = (int)s.cell.params.size();
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            const Species& sp = s.species[i];
+            if (sp.kind != 0) continue;
+            for (int sl = 1; sl <= sp.Ns + 1; ++sl) {
+                const int r = sb[i].row0 + sl - 1;
+                StressLane<T>& L = slanes[r];
+                L.nx = nx;
+                L.ny = ny;
+                L.nz = nz;
+                L.q = sb[i].q + (size_t)(sl - 1) * n;
+                L.qc = sb[i].qc ? sb[i].qc + (size_t)(sl - 1) * n
+                                : sb[i].q + (size_t)(sp.Ns + 1 - sl) * n;
+                L.out = r;
+                L.nparams = np;
+                L.dim = s.cell.dim;
+                std::memset(L.dGG, 0, sizeof(L.dGG));
+                for (int k = 0; k < 9 * np; ++k) L.dGG[k] = s.cell.dGG[k];
+                L.twx = twx;
+                L.twy = twy;
+                L.twz = twz;
+                L.px = px;
+                L.py = py;
+                L.pz = pz;
+                if (!fused) {
+                    const int c = r % schunk;
+                    L.kq = kstress + (size_t)(2 * c) * nk;
+                    L.kqc = kstress + (size_t)(2 * c + 1) * nk;
+                }
+            }
+        }
+        if (nrows) upload(d_slanes, slanes.data(), sizeof(StressLane<T>) * nrows);
+        slanes_cell_version = s.cell.version;
+    }
+
+    void stress_rows(System& s, std::vector<double>& rows) override {
+        if (s.dirty_species || sb.size() != s.species.size()) build();
+        if (slanes_cell_version != s.cell.version || (int)slanes.size() != nrows) build_stress();
+        rows.assign((size_t)nrows * PF_MAXPAR, 0.0);
+        if (!nrows) return;
+        hipStream_t st = s.stream;
+        const bool fused = planar && 2 * lds_plane <= 160 * 1024;
+        if (fused) {
+            launch_stress_planar<T>(d_slanes, nrows, srows, 2 * lds_plane, 256, st);
+        } else {
+            const int ny = s.dims[1], nz = s.dims[2];
+            const int per = ny * ntiles;
+            for (int r0 = 0; r0 < nrows; r0 += schunk) {
+                const int cnt = std::min(schunk, nrows - r0);
+                launch_stress_slab<T>(d_slanes + r0, cnt, nz, lds_plane, 256, st);
+                launch_stress_pencil<T>(d_slanes + r0, cnt, ny, ntiles, tile, spart,
+                                        2 * lds_pencil, 256, st);
+            }
+            launch_segsum(spart, nrows, per, PF_MAXPAR, PF_MAXPAR, srows, st);
+        }
+        PF_HIP(hipMemcpyAsync(rows.data(), srows, sizeof(double) * nrows * PF_MAXPAR,
+                              hipMemcpyDeviceToHost, st));
+        PF_HIP(hipStreamSynchronize(st));
+    }
+
+    void get_propagator(System& s, int isp, int slice, int backward, double* host) override {
+        if (sb.size() != s.species.size() || s.species[isp].kind != 0)
+            throw Error(-2, "no propagator for this species (run density first)");
+        const Species& sp = s.species[isp];
+        if (slice < 1 || slice > sp.Ns + 1) throw Error(-2, "contour index out of range");
+        const T* src;
+        if (!backward)
+            src = sb[isp].q + (size_t)(slice - 1) * s.n;
+        else if (sb[isp].qc)
+            src = sb[isp].qc + (size_t)(slice - 1) * s.n;
+        else
+            src = sb[isp].q + (size_t)(sp.Ns + 1 - slice) * s.n;
+        std::vector<T> tmp(s.n);
+        PF_HIP(hipMemcpyAsync(tmp.data(), src, sizeof(T) * s.n, hipMemcpyDeviceToHost, s.stream));
+        PF_HIP(hipStreamSynchronize(s.stream));
+        for (size_t i = 0; i < s.n; ++i) host[i] = (double)tmp[i];
+    }
+};
+
+std::unique_ptr<EngineBase> make_engine(System& s) {
+    if (s.precision == 0) return std::unique_ptr<EngineBase>(new Engine<double>(s));
+    return std::unique_ptr<EngineBase>(new Engine<float>(s));
+}
+
+}  // namespace pf

@@ -1,0 +1,629 @@
+// Propagator kernels: LDS-resident Stockham transforms fused with the Strang-step
+// multiplies of the pseudo-spectral MDE solver, and the k-space stress reductions.
+//
+// Reference semantics (PolyFields.jl): one RK2 step is
+//   qr = LW .* q_s ; qk = rfft(qr) ; qk .*= LD ; qr = irfft(qk) (1/n) ; q_{s+1} = LW .* qr
+// (src/system/mde.jl:76-81) with LW = exp(-ds/2 v w), LD = exp(-ds b^2/6 k^2)
+// (src/system/mde.jl:47-54) and k^2 on folded wave indices (src/system/cell.jl:268-285).
+//
+// Layout: real grids are x fastest, q[(z*ny + y)*nx + x]; the half spectrum keeps
+// kx = 0..nx/2 fastest, k[(kz*ny + ky)*(nx/2+1) + kx] (the reference's FFTW r2c layout).
+// The x transform of a real row of length nx = 2M runs as a length-M complex transform
+// of the packed pairs (x[2m], x[2m+1]) followed by the split step that yields the M+1
+// half-spectrum entries; the inverse runs the split step backwards.
+#include "pf_internal.h"
+
+namespace pf {
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
+// multiply by +i (INV) or -i (forward)
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> mul_i(cpx<T> a) {
+    return INV ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+}
+
+template <typename T>
+__device__ __forceinline__ T exp_t(double a);
+template <>
+__device__ __forceinline__ double exp_t<double>(double a) { return exp(a); }
+template <>
+__device__ __forceinline__ float exp_t<float>(double a) { return expf((float)a); }
+
+// Small DFTs of length R in registers.  Forward uses exp(-2 pi i/R), inverse exp(+...).
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+    if constexpr (R == 2) {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const T c = T(-0.5);
+        const T s = T(0.86602540378443864676372317075294);
+        cpx<T> t1 = cadd(v[1], v[2]);
+        cpx<T> d = csub(v[1], v[2]);
+        cpx<T> m = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+        cpx<T> sd = mul_i<T, INV>(cpx<T>{s * d.x, s * d.y});
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, sd);
+        v[2] = csub(m, sd);
+    } else if constexpr (R == 4) {
+        cpx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        cpx<T> b0 = cadd(v[1], v[3]), b1 = mul_i<T, INV>(csub(v[1], v[3]));
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    }
+}
+
+// One Stockham pass of radix R over `nlines` independent lines held in LDS.  Element e
+// of line l sits at buf[l*ls + e*es].  Each thread loads all butterflies it owns into
+// registers, the workgroup synchronises, and the outputs are written back into the same
+// buffer in autosorted order.  Lines are processed in rounds that fit the register
+// budget; a round always covers whole lines, so later rounds read untouched data.
+// Twiddle w_N^m is tw[m*twstride].
+template <typename T, int R, bool INV>
+__device__ void fft_pass(cpx<T>* buf, int N, int Ns, int nlines, int es, int ls,
+                         const cpx<T>* __restrict__ tw, int twstride) {
+    constexpr int MAXB = (R == 2) ? 4 : 2;
+    const int nb = N / R;
+    const int nthr = blockDim.x;
+    const int lpr = max(1, (nthr * MAXB) / nb);
+    const int twmul = (N / (Ns * R)) * twstride;
+    for (int l0 = 0; l0 < nlines; l0 += lpr) {
+        const int total = min(lpr, nlines - l0) * nb;
+        cpx<T> v[MAXB][R];
+#pragma unroll
+        for (int i = 0; i < MAXB; ++i) {
+            const int b = threadIdx.x + i * nthr;
+            if (b < total) {
+                const int line = l0 + b / nb;
+                const int j = b - (b / nb) * nb;
+                const cpx<T>* src = buf + line * ls;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = src[(j + r * nb) * es];
+                if (Ns > 1) {
+                    const int k = j % Ns;
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        cpx<T> w = tw[k * r * twmul];
+                        if (INV) w.y = -w.y;
+                        v[i][r] = cmul(v[i][r], w);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < MAXB; ++i) {
+            const int b = threadIdx.x + i * nthr;
+            if (b < total) {
+                const int line = l0 + b / nb;
+                const int j = b - (b / nb) * nb;
+                const int k = j % Ns;
+                dft<T, R, INV>(v[i]);
+                const int d = (j / Ns) * Ns * R + k;
+                cpx<T>* dst = buf + line * ls;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[(d + r * Ns) * es] = v[i][r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, bool INV>
+__device__ void lds_fft(cpx<T>* buf, const FftPlan& p, int nlines, int es, int ls,
+                        const cpx<T>* __restrict__ tw, int twstride) {
+    const int N = p.n;
+    int Ns = 1;
+    for (int ip = 0; ip < p.npass; ++ip) {
+        const int R = p.radix[ip];
+        if (R == 4)
+            fft_pass<T, 4, INV>(buf, N, Ns, nlines, es, ls, tw, twstride);
+        else if (R == 2)
+            fft_pass<T, 2, INV>(buf, N, Ns, nlines, es, ls, tw, twstride);
+        else
+            fft_pass<T, 3, INV>(buf, N, Ns, nlines, es, ls, tw, twstride);
+        Ns *= R;
+    }
+}
+
+// Split step after the length-M transform of packed rows: Z -> X_k, k = 0..M.
+// X_k = E + W^k O and X_{M-k} = conj(E - W^k O), E = (Z_k + Z*_{M-k})/2,
+// O = -i (Z_k - Z*_{M-k})/2, W = exp(-2 pi i / 2M).
+template <typename T>
+__device__ void rows_r2c_split(cpx<T>* S, int M, int nrows, int ld,
+                               const cpx<T>* __restrict__ twN) {
+    const int half = M / 2 + 1;
+    for (int i = threadIdx.x; i < nrows * half; i += blockDim.x) {
+        const int y = i / half, k = i - y * half;
+        cpx<T>* row = S + y * ld;
+        const int km = (M - k) % M;
+        const cpx<T> zk = row[k], zm = row[km];
+        const cpx<T> e = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+        const cpx<T> d = {zk.x - zm.x, zk.y + zm.y};        // Z_k - conj(Z_{M-k})
+        const cpx<T> o = {T(0.5) * d.y, T(-0.5) * d.x};      // -i d / 2
+        const cpx<T> wo = cmul(twN[k], o);
+        row[k] = cadd(e, wo);
+        if (M - k != k) row[M - k] = cconj(csub(e, wo));
+    }
+    __syncthreads();
+}
+
+// Inverse split step: X_k (k = 0..M, Hermitian row) -> Z_k, k = 0..M-1, with the
+// imaginary parts of X_0 and X_M dropped as a c2r transform does.
+template <typename T>
+__device__ void rows_c2r_split(cpx<T>* S, int M, int nrows, int ld,
+                               const cpx<T>* __restrict__ twN) {
+    const int half = M / 2 + 1;
+    for (int i = threadIdx.x; i < nrows * half; i += blockDim.x) {
+        const int y = i / half, k = i - y * half;
+        cpx<T>* row = S + y * ld;
+        cpx<T> xa = row[k], xb = row[M - k];
+        if (k == 0) {
+            xa.y = T(0);
+            xb.y = T(0);
+        }
+        const cpx<T> e = {T(0.5) * (xa.x + xb.x), T(0.5) * (xa.y - xb.y)};
+        const cpx<T> dd = {T(0.5) * (xa.x - xb.x), T(0.5) * (xa.y + xb.y)};
+        const cpx<T> o = cmul(dd, cconj(twN[k]));
+        row[k] = {e.x - o.y, e.y + o.x};                     // E + i O
+        if (k != 0 && M - k != k) row[M - k] = {e.x + o.y, -e.y + o.x};   // conj(E) + i conj(O)
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int fold(int i, int n) { return (2 * i <= n) ? i : n - i; }
+
+__device__ __forceinline__ double ksq_at(const double* G, double jx, double jy, double jz) {
+    const double kx = G[0] * jx + G[1] * jy + G[2] * jz;
+    const double ky = G[3] * jx + G[4] * jy + G[5] * jz;
+    const double kz = G[6] * jx + G[7] * jy + G[8] * jz;
+    return kx * kx + ky * ky + kz * kz;
+}
+
+template <typename T>
+__device__ __forceinline__ int step_block(const Lane<T>* L, int t, int* s_in) {
+    if (L->dir > 0) {
+        const int s = 1 + t;
+        *s_in = s;
+        int b = 0;
+        while (b + 1 < L->nblk && L->begin[b + 1] <= s) ++b;
+        return b;
+    }
+    const int s = L->nsteps + 1 - t;
+    *s_in = s;
+    int b = L->nblk - 1;
+    while (b > 0 && L->begin[b] + 1 > s) --b;
+    return b;
+}
+
+// x-forward then y-forward of a plane held as packed real rows in S.
+template <typename T>
+__device__ void plane_forward(cpx<T>* S, int M, int ny, int ld, const FftPlan& px,
+                              const FftPlan& py, const cpx<T>* twx, const cpx<T>* twy) {
+    lds_fft<T, false>(S, px, ny, 1, ld, twx, 2);
+    rows_r2c_split(S, M, ny, ld, twx);
+    if (ny > 1) lds_fft<T, false>(S, py, M + 1, ld, 1, twy, 1);
+}
+
+template <typename T>
+__device__ void plane_inverse(cpx<T>* S, int M, int ny, int ld, const FftPlan& px,
+                              const FftPlan& py, const cpx<T>* twx, const cpx<T>* twy) {
+    if (ny > 1) lds_fft<T, true>(S, py, M + 1, ld, 1, twy, 1);
+    rows_c2r_split(S, M, ny, ld, twx);
+    lds_fft<T, true>(S, px, ny, 1, ld, twx, 2);
+}
+
+// ---------------------------------------------------------------------------------
+// 1-D / 2-D lanes: a workgroup owns one lane's whole plane and advances it through
+// contour steps [t0, t1) without leaving LDS; every q_s is written to the stack.
+template <typename T>
+__global__ void __launch_bounds__(256) k_planar_steps(const Lane<T>* __restrict__ lanes, int t0,
+                                                      int t1) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* S = reinterpret_cast<cpx<T>*>(smem_raw);
+    const Lane<T>* L = lanes + blockIdx.x;
+    const int nx = L->nx, ny = L->ny, M = nx >> 1, ld = M + 1, n = nx * ny;
+    const int tend = min(t1, L->nsteps);
+    if (t0 >= tend) return;
+    const int np = n >> 1;
+    const T scale = T(2.0 / (double)n);
+    int s_in;
+    step_block(L, t0, &s_in);
+    {
+        const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n);
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            S[y * ld + (i - y * M)] = qin[i];
+        }
+    }
+    for (int t = t0; t < tend; ++t) {
+        const int b = step_block(L, t, &s_in);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b]);
+        __syncthreads();
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            cpx<T>& v = S[y * ld + (i - y * M)];
+            const cpx<T> w = lw[i];
+            v.x *= w.x;
+            v.y *= w.y;
+        }
+        __syncthreads();
+        plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        const double mc = -L->ldc[b];
+        const int nkp = ny * (M + 1);
+        for (int i = threadIdx.x; i < nkp; i += blockDim.x) {
+            const int y = i / (M + 1), kx = i - y * (M + 1);
+            const T f = exp_t<T>(mc * ksq_at(L->G, (double)kx, (double)fold(y, ny), 0.0));
+            cpx<T>& v = S[y * ld + kx];
+            v.x *= f;
+            v.y *= f;
+        }
+        __syncthreads();
+        plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n);
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            cpx<T>& v = S[y * ld + (i - y * M)];
+            const cpx<T> w = lw[i];
+            const cpx<T> o = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+            v = o;
+            qout[i] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// 3-D lanes, xy-slab pass.  SLAB_FWD: LW_b(t) * q_{s_in(t)} -> x,y forward -> kbuf.
+// SLAB_INVFWD: kbuf -> y,x inverse -> scale, * LW_b(t) -> q_{s_out(t)}; then, when a
+// step t+1 exists, * LW_b(t+1) -> x,y forward -> kbuf.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_slab(const Lane<T>* __restrict__ lanes, int nz, int t) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* S = reinterpret_cast<cpx<T>*>(smem_raw);
+    const int lane = blockIdx.x / nz;
+    const int z = blockIdx.x - lane * nz;
+    const Lane<T>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    const int nx = L->nx, ny = L->ny, M = nx >> 1, ld = M + 1;
+    const int plane = nx * ny, n = plane * nz, np = plane >> 1;
+    const int nxh = M + 1, kplane = nxh * ny;
+    cpx<T>* kb = L->kbuf + (size_t)z * kplane;
+    int s_in;
+    int b = step_block(L, t, &s_in);
+    bool fwd = (MODE == SLAB_FWD);
+    if (MODE == SLAB_INVFWD) {
+        for (int i = threadIdx.x; i < kplane; i += blockDim.x) {
+            const int y = i / nxh;
+            S[y * ld + (i - y * nxh)] = kb[i];
+        }
+        __syncthreads();
+        plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        const T scale = T(2.0 / (double)n);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * plane);
+        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
+                                                 (size_t)z * plane);
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            cpx<T>& v = S[y * ld + (i - y * M)];
+            const cpx<T> w = lw[i];
+            const cpx<T> o = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+            v = o;
+            qout[i] = o;
+        }
+        if (t + 1 >= L->nsteps) return;
+        b = step_block(L, t + 1, &s_in);
+        fwd = true;
+        __syncthreads();
+        const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * plane);
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            cpx<T>& v = S[y * ld + (i - y * M)];
+            const cpx<T> w = lw2[i];
+            v.x *= w.x;
+            v.y *= w.y;
+        }
+    } else {
+        const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
+                                                            (size_t)z * plane);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * plane);
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            const cpx<T> q = qin[i], w = lw[i];
+            S[y * ld + (i - y * M)] = {q.x * w.x, q.y * w.y};
+        }
+    }
+    if (fwd) {
+        __syncthreads();
+        plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        for (int i = threadIdx.x; i < kplane; i += blockDim.x) {
+            const int y = i / nxh;
+            kb[i] = S[y * ld + (i - y * nxh)];
+        }
+    }
+}
+
+// 3-D lanes, z-pencil pass: forward z, * LD_b(t), inverse z, for `tile` kx columns of
+// one ky row.
+template <typename T>
+__global__ void __launch_bounds__(256) k_pencil(const Lane<T>* __restrict__ lanes, int ny,
+                                                int ntiles, int tile, int t) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
+    const int per = ny * ntiles;
+    const int lane = blockIdx.x / per;
+    const int r = blockIdx.x - lane * per;
+    const int ky = r / ntiles, tl = r - ky * ntiles;
+    const Lane<T>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    const int nz = L->nz, nxh = (L->nx >> 1) + 1;
+    const int kx0 = tl * tile;
+    const int tw = min(tile, nxh - kx0);
+    int s_in;
+    const int b = step_block(L, t, &s_in);
+    cpx<T>* kb = L->kbuf + (size_t)ky * nxh + kx0;
+    const size_t zstride = (size_t)ny * nxh;
+    for (int i = threadIdx.x; i < nz * tw; i += blockDim.x) {
+        const int z = i / tw, c = i - z * tw;
+        P[z * tile + c] = kb[z * zstride + c];
+    }
+    __syncthreads();
+    lds_fft<T, false>(P, L->pz, tw, tile, 1, L->twz, 1);
+    const double mc = -L->ldc[b];
+    const double jy = (double)fold(ky, ny);
+    for (int i = threadIdx.x; i < nz * tw; i += blockDim.x) {
+        const int z = i / tw, c = i - z * tw;
+        const T f = exp_t<T>(mc * ksq_at(L->G, (double)(kx0 + c), jy, (double)fold(z, nz)));
+        cpx<T>& v = P[z * tile + c];
+        v.x *= f;
+        v.y *= f;
+    }
+    __syncthreads();
+    lds_fft<T, true>(P, L->pz, tw, tile, 1, L->twz, 1);
+    for (int i = threadIdx.x; i < nz * tw; i += blockDim.x) {
+        const int z = i / tw, c = i - z * tw;
+        kb[z * zstride + c] = P[z * tile + c];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Stress: sum_k c_k Re(qc^_k) Re(q^_k) dksq_theta(k), c_0 = 1 and every other half-
+// spectrum entry doubled (src/species/multiblock.jl:276-282).
+
+__device__ __forceinline__ double dksq_at(const double* dGG, int np, int k, int dim, double j0,
+                                          double j1, double j2) {
+    const double j[3] = {j0, j1, j2};
+    double acc = 0.0;
+    for (int l = 0; l < dim; ++l)
+        for (int m = 0; m < dim; ++m) acc += j[l] * j[m] * dGG[(l * 3 + m) * np + k];
+    return acc;
+}
+
+template <int NV>
+__device__ void block_sum(double* v, double* out) {
+    // wavefront reduction then across waves through LDS; result in out[0..NV) of thread 0
+    __shared__ double red[PF_MAXPAR * 16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) red[k * 16 + wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += red[k * 16 + w];
+            out[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ void stress_accumulate(const cpx<T>* Sq, const cpx<T>* Sc, int ldq, int nkx,
+                                  int nrows, int ny, int nz, int row0, int kz_fixed,
+                                  int kx0, const StressLane<T>* L, double* acc) {
+    // rows index ky (planar) or z (pencil, with kx0 offset)
+    const int np = L->nparams;
+    for (int i = threadIdx.x; i < nrows * nkx; i += blockDim.x) {
+        const int rr = i / nkx, c = i - rr * nkx;
+        int kx, ky, kz;
+        if (kz_fixed >= 0) {
+            kx = c; ky = rr; kz = kz_fixed;
+        } else {
+            kx = kx0 + c; ky = row0; kz = rr;
+        }
+        const size_t flat = ((size_t)kz * ny + ky) * (size_t)((L->nx >> 1) + 1) + kx;
+        const double wgt = (flat == 0) ? 1.0 : 2.0;
+        const double a = (double)Sc[rr * ldq + c].x;
+        const double bq = (double)Sq[rr * ldq + c].x;
+        const double jx = kx, jy = fold(ky, ny), jz = fold(kz, nz);
+        for (int k = 0; k < np; ++k)
+            acc[k] += (wgt * a) * (bq * dksq_at(L->dGG, np, k, L->dim, jx, jy, jz));
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stress_planar(const StressLane<T>* __restrict__ lanes,
+                                                       double* partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const StressLane<T>* L = lanes + blockIdx.x;
+    const int nx = L->nx, ny = L->ny, M = nx >> 1, ld = M + 1, np2 = (nx * ny) >> 1;
+    cpx<T>* Sq = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* Sc = Sq + ny * ld;
+    const cpx<T>* q = reinterpret_cast<const cpx<T>*>(L->q);
+    const cpx<T>* qc = reinterpret_cast<const cpx<T>*>(L->qc);
+    for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+        const int y = i / M, o = y * ld + (i - y * M);
+        Sq[o] = q[i];
+        Sc[o] = qc[i];
+    }
+    __syncthreads();
+    plane_forward(Sq, M, ny, ld, L->px, L->py, L->twx, L->twy);
+    plane_forward(Sc, M, ny, ld, L->px, L->py, L->twx, L->twy);
+    double acc[PF_MAXPAR] = {0, 0, 0, 0, 0, 0};
+    stress_accumulate(Sq, Sc, ld, M + 1, ny, ny, 1, 0, 0, 0, L, acc);
+    double out[PF_MAXPAR];
+    block_sum<PF_MAXPAR>(acc, out);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < L->nparams; ++k) partial[(size_t)L->out * PF_MAXPAR + k] = out[k];
+}
+
+// 3-D stress, slab part: plain r2c of q_s and q+_s planes (no LW) into kq / kqc.
+template <typename T>
+__global__ void __launch_bounds__(256) k_stress_slab(const StressLane<T>* __restrict__ lanes,
+                                                     int nz) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<T>* S = reinterpret_cast<cpx<T>*>(smem_raw);
+    const int lane = blockIdx.x / (2 * nz);
+    const int r = blockIdx.x - lane * 2 * nz;
+    const int which = r / nz, z = r - which * nz;
+    const StressLane<T>* L = lanes + lane;
+    const int nx = L->nx, ny = L->ny, M = nx >> 1, ld = M + 1, plane = nx * ny, np2 = plane >> 1;
+    const int nxh = M + 1, kplane = nxh * ny;
+    const cpx<T>* src = reinterpret_cast<const cpx<T>*>((which ? L->qc : L->q) + (size_t)z * plane);
+    cpx<T>* dst = (which ? L->kqc : L->kq) + (size_t)z * kplane;
+    for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+        const int y = i / M;
+        S[y * ld + (i - y * M)] = src[i];
+    }
+    __syncthreads();
+    plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+    for (int i = threadIdx.x; i < kplane; i += blockDim.x) {
+        const int y = i / nxh;
+        dst[i] = S[y * ld + (i - y * nxh)];
+    }
+}
+
+// 3-D stress, pencil part: z-forward of both spectra and the weighted reduction.
+template <typename T>
+__global__ void __launch_bounds__(256) k_stress_pencil(const StressLane<T>* __restrict__ lanes,
+                                                       int ny, int ntiles, int tile,
+                                                       double* partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int per = ny * ntiles;
+    const int lane = blockIdx.x / per;
+    const int r = blockIdx.x - lane * per;
+    const int ky = r / ntiles, tl = r - ky * ntiles;
+    const StressLane<T>* L = lanes + lane;
+    const int nz = L->nz, nxh = (L->nx >> 1) + 1;
+    cpx<T>* Pq = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* Pc = Pq + nz * tile;
+    const int kx0 = tl * tile;
+    const int tw = min(tile, nxh - kx0);
+    const size_t zstride = (size_t)ny * nxh;
+    const cpx<T>* a = L->kq + (size_t)ky * nxh + kx0;
+    const cpx<T>* c = L->kqc + (size_t)ky * nxh + kx0;
+    for (int i = threadIdx.x; i < nz * tw; i += blockDim.x) {
+        const int z = i / tw, cc = i - z * tw;
+        Pq[z * tile + cc] = a[z * zstride + cc];
+        Pc[z * tile + cc] = c[z * zstride + cc];
+    }
+    __syncthreads();
+    lds_fft<T, false>(Pq, L->pz, tw, tile, 1, L->twz, 1);
+    lds_fft<T, false>(Pc, L->pz, tw, tile, 1, L->twz, 1);
+    double acc[PF_MAXPAR] = {0, 0, 0, 0, 0, 0};
+    stress_accumulate(Pq, Pc, tile, tw, nz, ny, nz, ky, -1, kx0, L, acc);
+    double out[PF_MAXPAR];
+    block_sum<PF_MAXPAR>(acc, out);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < L->nparams; ++k)
+            partial[((size_t)L->out * per + r) * PF_MAXPAR + k] = out[k];
+}
+
+// ---------------------------------------------------------------------------------
+template <typename Kern>
+static void set_lds(Kern k, int bytes) {
+    if (bytes > 64 * 1024)
+        PF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+}
+
+template <typename T>
+void launch_planar_steps(const Lane<T>* d_lanes, int nlanes, int t0, int t1, int lds_bytes,
+                         int threads, hipStream_t st) {
+    set_lds(k_planar_steps<T>, lds_bytes);
+    hipLaunchKernelGGL(k_planar_steps<T>, dim3(nlanes), dim3(threads), lds_bytes, st, d_lanes,
+                       t0, t1);
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, int lds_bytes,
+                 int threads, hipStream_t st) {
+    if (mode == SLAB_FWD) {
+        set_lds(k_slab<T, SLAB_FWD>, lds_bytes);
+        hipLaunchKernelGGL((k_slab<T, SLAB_FWD>), dim3(nlanes * nz), dim3(threads), lds_bytes,
+                           st, d_lanes, nz, t);
+    } else {
+        set_lds(k_slab<T, SLAB_INVFWD>, lds_bytes);
+        hipLaunchKernelGGL((k_slab<T, SLAB_INVFWD>), dim3(nlanes * nz), dim3(threads),
+                           lds_bytes, st, d_lanes, nz, t);
+    }
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
+                   int lds_bytes, int threads, hipStream_t st) {
+    set_lds(k_pencil<T>, lds_bytes);
+    hipLaunchKernelGGL(k_pencil<T>, dim3(nlanes * ny * ntiles), dim3(threads), lds_bytes, st,
+                       d_lanes, ny, ntiles, tile, t);
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_stress_planar(const StressLane<T>* d_lanes, int nlanes, double* partial,
+                          int lds_bytes, int threads, hipStream_t st) {
+    set_lds(k_stress_planar<T>, lds_bytes);
+    hipLaunchKernelGGL(k_stress_planar<T>, dim3(nlanes), dim3(threads), lds_bytes, st, d_lanes,
+                       partial);
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_stress_slab(const StressLane<T>* d_lanes, int nlanes, int nz, int lds_bytes,
+                        int threads, hipStream_t st) {
+    set_lds(k_stress_slab<T>, lds_bytes);
+    hipLaunchKernelGGL(k_stress_slab<T>, dim3(nlanes * 2 * nz), dim3(threads), lds_bytes, st,
+                       d_lanes, nz);
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_stress_pencil(const StressLane<T>* d_lanes, int nlanes, int ny, int ntiles,
+                          int tile, double* partial, int lds_bytes, int threads,
+                          hipStream_t st) {
+    set_lds(k_stress_pencil<T>, lds_bytes);
+    hipLaunchKernelGGL(k_stress_pencil<T>, dim3(nlanes * ny * ntiles), dim3(threads), lds_bytes,
+                       st, d_lanes, ny, ntiles, tile, partial);
+    PF_HIP(hipGetLastError());
+}
+
+#define PF_INST(T)                                                                          \
+    template void launch_planar_steps<T>(const Lane<T>*, int, int, int, int, int, hipStream_t); \
+    template void launch_slab<T>(const Lane<T>*, int, int, int, int, int, int, hipStream_t);  \
+    template void launch_pencil<T>(const Lane<T>*, int, int, int, int, int, int, int,         \
+                                   hipStream_t);                                            \
+    template void launch_stress_planar<T>(const StressLane<T>*, int, double*, int, int,       \
+                                          hipStream_t);                                     \
+    template void launch_stress_slab<T>(const StressLane<T>*, int, int, int, int, hipStream_t); \
+    template void launch_stress_pencil<T>(const StressLane<T>*, int, int, int, int, double*,  \
+                                          int, int, hipStream_t);
+PF_INST(float)
+PF_INST(double)
+
+}  // namespace pf

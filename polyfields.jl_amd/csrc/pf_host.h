@@ -1,0 +1,205 @@
+// Host-side state of a device-resident FieldSystem and its updaters.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pf_internal.h"
+
+namespace pf {
+
+// --- geometry (pf_cell.cpp) ---------------------------------------------------------
+struct Cell {
+    int dim = 1;
+    std::string crystal;
+    std::vector<double> params;
+    double R[9] = {0}, G[9] = {0}, volume = 0.0;
+    std::vector<double> dR, dG, dRR, dGG;   // [(i*3+j)*np + k]
+    long version = 0;                      // bumped by update()
+    void basis();
+    void dbasis();
+    void ddbasis();
+    void update();                         // src/system/cell.jl:96-109 (basis part)
+};
+int crystal_nparams(int dim, const std::string& crystal);
+void chain_grid(const std::vector<double>& N_block, int Ns, double* N_out,
+                std::vector<int>& begin, std::vector<double>& ds_block);
+bool lu_inverse(int n, const double* U, double* inv);
+void matvec(int n, const double* M, const double* v, double* out);
+
+// --- species ------------------------------------------------------------------------
+struct Species {
+    int kind = 0;                    // 0 linear chain (Multiblock), 1 point particle
+    std::vector<int> block_slot;     // monomer slot per block
+    std::vector<int> N_block;
+    std::vector<double> b_block, vol_block, Nref_block, ds_block, f_block;
+    std::vector<int> begin;          // 1-based block starts, nblk+1 entries
+    int Ns = 0;
+    double N = 0, Nref = 0;
+    std::vector<int> slots;          // monomer slots present (ascending id)
+    int nblk() const { return (int)N_block.size(); }
+    double* rho_sp = nullptr;        // device [slots.size()][n]
+};
+
+class EngineBase;
+
+// A FieldSystem with all grids resident on one GPU (src/system/system.jl:22-49).
+struct System {
+    int dims[3];
+    size_t n = 0;
+    int dim = 0;
+    int precision = 0;               // 0: FP64 everywhere, 1: FP32 propagator + FP64 sums
+    int canonical = 1;
+    int compress_mode = 0;           // 0 strict, 1 compressible, 2 unconstrained
+    double inv_zeta = 0.0;
+    int device = 0;
+    Cell cell;
+
+    std::vector<int> mon_id;         // ascending monomer ids; slot = index
+    std::vector<double> mon_vol, mon_charge;
+    int has_flory = 0;
+    double chi[PF_MAXMON * PF_MAXMON] = {0};
+    int pair[PF_MAXMON * PF_MAXMON] = {0};
+    double u[PF_MAXMON] = {0};
+    int u_set[PF_MAXMON] = {0};
+
+    std::vector<Species> species;
+    std::vector<double> phi, mu;     // host mirrors (canonical phi fixed, grand mu fixed)
+    std::vector<double> Q;           // host mirror, refreshed by fetch_scalars()
+
+    // device grids, FP64, [slot][n]
+    double* w = nullptr;
+    double* pot = nullptr;
+    double* res = nullptr;
+    double* rho = nullptr;
+    DevScalars* sc = nullptr;
+    double* partial = nullptr;       // reduction scratch
+    size_t partial_len = 0;
+    double* sums = nullptr;          // small result area (64 doubles)
+    double* pinned = nullptr;        // pinned host staging (4096 doubles)
+    hipStream_t stream = nullptr;
+
+    std::unique_ptr<EngineBase> eng;
+    bool fields_set = false;
+    int dirty_species = 1;           // engine must rebuild lanes
+
+    System(int nx, int ny, int nz, int dim_, const std::string& crystal,
+           const std::vector<double>& params, int precision, int canonical, double inv_zeta,
+           int device);
+    ~System();
+
+    int nmon() const { return (int)mon_id.size(); }
+    int slot_of(int id) const;
+    void add_monomer(int id, double vol, double charge);
+    int add_chain(const std::vector<int>& mids, const std::vector<int>& N_block,
+                  const std::vector<double>& b_block, int Ns, double val);
+    int add_point(int mid, double val);
+    void set_flory(int a, int b, double chi);
+    void set_edwards(int a, double u);
+
+    ResidualArgs residual_args() const;
+    void upload_scalars();
+    void fetch_scalars();            // Q, phi, mu -> host (synchronises)
+
+    // reference operations
+    void validate();
+    void density();                  // density!(sys)
+    void residuals();                // residuals!(sys)
+    void sys_sums(double out[5]);    // [res^2, w^2, rho*w, flory, edwards]
+    double field_error();
+    double free_energy();
+    double free_energy_bulk();
+    std::vector<double> scfstress(int exact = 0);
+    void monomer_fractions(std::vector<double>& fr);
+    void sync();
+};
+
+class EngineBase {
+   public:
+    virtual ~EngineBase() = default;
+    virtual void density(System& s) = 0;
+    virtual void stress_rows(System& s, std::vector<double>& rows) = 0;   // [row][PF_MAXPAR]
+    virtual void get_propagator(System& s, int isp, int slice, int backward, double* host) = 0;
+    virtual void timing(int on) = 0;
+    virtual double timing_ms(int which, int* count) = 0;
+};
+std::unique_ptr<EngineBase> make_engine(System& s);
+
+// --- field updaters ------------------------------------------------------------------
+struct Updater {
+    virtual ~Updater() = default;
+    virtual void setup(System* s) = 0;
+    virtual void step() = 0;
+    System* sys = nullptr;
+};
+
+struct Anderson : Updater {
+    int nhist, iter_euler;
+    double lam_euler;
+    int iter = 0, iter_hist = 0;
+    double lam = 0.0;
+    bool is_setup = false;
+    double* hist_r = nullptr;        // (nhist+1) slots of nmon*n
+    double* hist_w = nullptr;
+    int base = 0;                    // physical slot of logical slot 0
+    std::vector<double> lastU, lastV, lastC;
+    Anderson(int nhist, int iter_euler, double lam_euler);
+    ~Anderson();
+    double* rslot(int logical) const;
+    double* wslot(int logical) const;
+    void setup(System* s) override;
+    void step() override;
+};
+
+struct Euler : Updater {
+    int pece;
+    double lam;
+    double* tmp = nullptr;
+    Euler(int pece, double lam) : pece(pece), lam(lam) {}
+    ~Euler();
+    void setup(System* s) override;
+    void step() override;
+};
+
+struct Momentum : Updater {
+    int method;                      // 0 MOM, 1 RMS, 2 ADAM, 3 NAG
+    double lam, gamma, beta1, beta2;
+    double *stepb = nullptr, *mt = nullptr, *vt = nullptr, *tmp = nullptr;
+    Momentum(int method, double lam, double gamma, double b1, double b2)
+        : method(method), lam(lam), gamma(gamma), beta1(b1), beta2(b2) {}
+    ~Momentum();
+    void setup(System* s) override;
+    void step() override;
+};
+
+struct Domain {
+    int nper, nskip;
+    double lam, pmax, rlxn;
+    System* sys = nullptr;
+    std::vector<double> oldparams, stepv, stress, dstress;
+    Domain(int nper, int nskip, double lam, double pmax, double rlxn)
+        : nper(nper), nskip(nskip), lam(lam), pmax(pmax), rlxn(rlxn) {}
+    void setup(System* s);
+    void scfstress_update();
+    double stress_error() const;
+    void step(int nsteps, double tol);
+    void dstress_update();
+};
+
+struct ScftOptions {
+    int nsteps, nsout, nsave;
+    double ftol, stol;
+};
+struct ScftResults {
+    double time;
+    int nsteps, converged;
+    double ferr, serr, fhelm;
+};
+ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o);
+
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+
+}  // namespace pf

@@ -1,0 +1,197 @@
+// Internal declarations shared by the HIP kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#define PF_MAXBLK 8      // blocks per chain
+#define PF_MAXPASS 16    // radix passes per transform
+#define PF_MAXMON 8      // monomer types per system
+#define PF_MAXPAR 6      // cell parameters
+#define PF_MAXHIST 16    // Anderson history depth
+
+namespace pf {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PF_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::pf::Error(-3, std::string("HIP error '") + hipGetErrorString(e_) +   \
+                                      "' at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+template <typename T>
+struct cpx {
+    T x, y;
+};
+
+// Radix plan for one transform length (product of radix[0..npass)).
+struct FftPlan {
+    int n;
+    int npass;
+    int8_t radix[PF_MAXPASS];
+};
+
+// One propagator lane: a chain sweep in one direction over one grid.  All pointers are
+// device pointers.  Contour indices s are 1-based as in the reference; slice s of the
+// stack starts at stack + (s-1)*n.
+template <typename T>
+struct Lane {
+    int nx, ny, nz;          // grid (x fastest)
+    int dir;                 // +1: s increases (q), -1: s decreases (q+)
+    int nsteps;              // Ns
+    int nblk;
+    int begin[PF_MAXBLK + 1];
+    const T* lw[PF_MAXBLK];  // exp(-ds_b/2 v w) per block, n entries
+    double ldc[PF_MAXBLK];   // ds_b b_b^2 / 6
+    T* stack;
+    cpx<T>* kbuf;            // nk complex scratch (3-D lanes)
+    double G[9];             // reciprocal basis, row-major G[i][j]
+    const cpx<T>* twx;       // exp(-2 pi i m / nx), m < nx
+    const cpx<T>* twy;       // exp(-2 pi i m / ny)
+    const cpx<T>* twz;       // exp(-2 pi i m / nz)
+    FftPlan px, py, pz;      // plans for nx/2, ny, nz
+};
+
+// Stress task: transforms of q_s and q+_s reduced against dksq.
+template <typename T>
+struct StressLane {
+    int nx, ny, nz;
+    const T* q;              // slice q_s
+    const T* qc;             // slice q+_s
+    int out;                 // row of the partial-sum table
+    int nparams, dim;
+    double dGG[3 * 3 * PF_MAXPAR];  // dGG[(l*3+m)*np + k]
+    const cpx<T>* twx;
+    const cpx<T>* twy;
+    const cpx<T>* twz;
+    FftPlan px, py, pz;
+    cpx<T>* kq;              // 3-D: nk scratch for q_s
+    cpx<T>* kqc;             // 3-D: nk scratch for q+_s
+};
+
+// Species entry for the Simpson density / Q kernels.
+template <typename T>
+struct SimpsonTask {
+    const T* q;              // forward stack
+    const T* qc;             // backward stack (nullptr: homopolymer, q+_s = q_{Ns+2-s})
+    int Ns, nblk;
+    int begin[PF_MAXBLK + 1];
+    double ds[PF_MAXBLK];
+    int blk_mon[PF_MAXBLK];  // monomer slot of each block
+    double Nref;
+    int isp;                 // species index (scalar slots)
+    double* rho_sp;          // species density per monomer slot: rho_sp[slot*n + i]
+    int nmon;
+};
+
+// System-level scalar slots kept on the device.
+struct DevScalars {
+    double Q[64];
+    double phi[64];
+    double mu[64];
+    double mean_w[PF_MAXMON];
+    double mean_r[PF_MAXMON];
+    double red[16];          // reduction results
+};
+
+// --- launchers (pf_fft.hip) -------------------------------------------------------
+template <typename T>
+void launch_planar_steps(const Lane<T>* d_lanes, int nlanes, int t0, int t1, int lds_bytes,
+                         int threads, hipStream_t st);
+template <typename T>
+void launch_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, int lds_bytes,
+                 int threads, hipStream_t st);
+template <typename T>
+void launch_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
+                   int lds_bytes, int threads, hipStream_t st);
+template <typename T>
+void launch_stress_planar(const StressLane<T>* d_lanes, int nlanes, double* partial,
+                          int lds_bytes, int threads, hipStream_t st);
+template <typename T>
+void launch_stress_slab(const StressLane<T>* d_lanes, int nlanes, int nz, int lds_bytes,
+                        int threads, hipStream_t st);
+template <typename T>
+void launch_stress_pencil(const StressLane<T>* d_lanes, int nlanes, int ny, int ntiles,
+                          int tile, double* partial, int lds_bytes, int threads,
+                          hipStream_t st);
+
+enum SlabMode { SLAB_FWD = 0, SLAB_INVFWD = 1 };
+
+// --- launchers (pf_kernels.hip) ---------------------------------------------------
+struct ResidualArgs {
+    int nmon;
+    double vol[PF_MAXMON];
+    int has_flory;
+    double chi[PF_MAXMON * PF_MAXMON];   // symmetric
+    int pair[PF_MAXMON * PF_MAXMON];     // 1 where a chi pair is set
+    double u[PF_MAXMON];                 // Edwards excluded volume
+    int u_set[PF_MAXMON];
+    int compress;                        // 0 strict, 1 compressible, 2 off
+    double inv_zeta;
+    int canonical;
+    double* w;                           // fields     [nmon][n]
+    double* pot;                         // potentials [nmon][n]
+    double* res;                         // residuals  [nmon][n]
+    const double* rho;                   // densities  [nmon][n]
+};
+
+struct PoolArgs {
+    int nmon;
+    int nterm[PF_MAXMON];
+    const double* src[PF_MAXMON][8];
+    int isp[PF_MAXMON][8];
+    double* dst;
+};
+
+struct HistPtrs {
+    const double* p[PF_MAXHIST];
+};
+
+struct MixCoef {
+    double C[PF_MAXHIST];
+};
+
+int reduce_parts(size_t n);
+template <typename T>
+void launch_lw_table(T* out, const double* w, double coef, size_t n, hipStream_t st);
+template <typename T>
+void launch_fill(T* p, T v, size_t n, hipStream_t st);
+template <typename T>
+void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t n, double* partial,
+                      int nparts, DevScalars* sc, hipStream_t st);
+template <typename T>
+void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t n, DevScalars* sc,
+                    hipStream_t st);
+void launch_point_density(const double* w, double vol, size_t n, double* rho_sp,
+                          double* partial, int nparts, DevScalars* sc, int isp,
+                          hipStream_t st);
+void launch_muphi(DevScalars* sc, int nsp, int canonical, hipStream_t st);
+void launch_pool(const PoolArgs& a, size_t n, DevScalars* sc, hipStream_t st);
+void launch_residuals(const ResidualArgs& a, size_t n, double* partial, int nparts,
+                      double* sums, hipStream_t st);
+void launch_sys_reduce(const ResidualArgs& a, size_t n, double* partial, int nparts,
+                       double* out, hipStream_t st);
+void launch_final_reduce(const double* partial, int nparts, int nvals, double* out,
+                         hipStream_t st);
+void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
+void launch_axpy(double* w, const double* r, double lam, size_t n, hipStream_t st);
+void launch_gram(int h, const double* r, const HistPtrs& hp, size_t N, double* partial,
+                 int nparts, double* out, hipStream_t st);
+void launch_anderson_mix(int h, double* w, const double* r, const HistPtrs& fh,
+                         const HistPtrs& rh, const double* f_iter, const MixCoef& c, double lam,
+                         size_t N, double* f_out, hipStream_t st);
+void launch_momentum(int method, double* mt, double* vt, double* step, const double* r,
+                     double b1, double b2, size_t N, hipStream_t st);
+void launch_segsum(const double* in, int nseg, int nparts, int stride, int nvals, double* out,
+                   hipStream_t st);
+
+}  // namespace pf

@@ -1,0 +1,192 @@
+"""HIP path (through the C ABI) against the FP64 CPU oracle on the same inputs.
+
+FP64 ("plain") mode: densities, partition functions, stress, residuals and free energy
+agree to rounding (the transforms differ in algorithm, not in precision); bounds are
+written per check.  FP32 mode reports its error against FP64 with its own bound."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import cos_fields, diblock_system, load_seed
+
+pytestmark = pytest.mark.gpu
+
+P = pytest.importorskip("polyfields_hip")
+
+
+def both(build, fields, precision="fp64"):
+    so = build(O)
+    sg = build(P, precision)
+    O.fieldinit(so[0], fields=fields)
+    P.fieldinit(sg[0], fields=fields)
+    return so, sg
+
+
+def rand_fields(shape, mids=(1, 2), scale=0.3, seed=0):
+    rng = np.random.default_rng(seed)
+    return {m: scale * rng.standard_normal(shape) for m in mids}
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+CASES = [
+    ("lam32", (32, 1, 1), 1, "lamellar", (10.0,), dict(f=0.5)),
+    ("hex16", (16, 16, 1), 2, "hexagonal", (6.0,), dict(f=0.35)),
+    ("rect24x12", (24, 12, 1), 2, "rectangular", (7.0, 4.0), dict(f=0.3)),
+    ("cubic8", (8, 8, 8), 3, "cubic", (4.0,), dict(f=0.35)),
+    ("ortho12x6x6", (12, 6, 6), 3, "orthorhombic", (5.0, 4.0, 3.5), dict(f=0.4)),
+    ("tric8x6x4", (8, 6, 4), 3, "triclinic", (6.0, 5.0, 4.0, 1.3, 1.4, 1.2), dict(f=0.5)),
+]
+
+
+@pytest.mark.parametrize("name,dims,dim,crystal,params,kw", CASES, ids=[c[0] for c in CASES])
+def test_density_stress_residuals_fp64(name, dims, dim, crystal, params, kw):
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, dims, dim, crystal, params, Ns=40, **kw, **extra)
+
+    shape = (dims[2], dims[1], dims[0])
+    (so, _, co), (sg, _, cg) = both(build, rand_fields(shape))
+    assert cg.Q == pytest.approx(co.Q, rel=1e-12)
+    for s in (0, 1, co.Ns // 2, co.Ns):
+        assert rel(cg.q[s], co.q[s]) < 1e-12
+        assert rel(cg.qc[s], co.qc[s]) < 1e-12
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 1e-12
+        assert rel(sg.residuals[m], so.residuals[m]) < 1e-11
+        assert rel(sg.fields[m], so.fields[m]) < 1e-14
+    assert sg.field_error() == pytest.approx(so.field_error(), rel=1e-11)
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12, abs=1e-14)
+    np.testing.assert_allclose(sg.scfstress(), so.scfstress(), rtol=1e-10, atol=1e-15)
+
+
+def test_blend_point_homopolymers_fp64():
+    # Point + two homopolymers with v = 2 and a chi triangle (examples.ipynb blend).
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        cell = mod.UnitCell(1, "lamellar", 20.0)
+        a, b, c = mod.Monomer(1, vol=2.0), mod.Monomer(2, vol=2.0), mod.Monomer(3, vol=1.0)
+        sys = mod.FieldSystem((64, 1, 1), cell, monomers=[a, b, c], **extra)
+        sys.add_species(mod.Point(c), 0.4)
+        sys.add_species(mod.Homopolymer(a, 100, 1.0), 0.3)
+        sys.add_species(mod.Homopolymer(b, 100, 1.0), 0.3)
+        itx = mod.FloryInteraction()
+        itx.set_interaction(1, 2, 0.5)
+        itx.set_interaction(1, 3, 1.25)
+        itx.set_interaction(2, 3, 1.25)
+        sys.add_interaction(itx)
+        return sys, cell, None
+
+    (so, _, _), (sg, _, _) = both(build, rand_fields((1, 1, 64), mids=(1, 2, 3), scale=0.05))
+    for m in (1, 2, 3):
+        assert rel(sg.density[m], so.density[m]) < 1e-12
+        assert rel(sg.residuals[m], so.residuals[m]) < 1e-10
+    for i in range(3):
+        assert sg.species[i].Q == pytest.approx(so.species[i].Q, rel=1e-12)
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
+    assert sg.free_energy_bulk() == pytest.approx(so.free_energy_bulk(), rel=1e-14)
+
+
+def test_anderson_trajectory_fp64():
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, (32, 32, 1), 2, "hexagonal", (9.0,), f=0.3, chi=0.25,
+                              Ns=50, **extra)
+
+    (so, _, _), (sg, _, _) = both(build, rand_fields((1, 32, 32), scale=0.5))
+    uo, ug = O.AndersonUpdater(nhist=5), P.AndersonUpdater(nhist=5)
+    uo.setup(so)
+    ug.setup(sg)
+    for it in range(14):
+        uo.step()
+        ug.step()
+        fo, fg = so.field_error(), sg.field_error()
+        eo, eg = so.free_energy(), sg.free_energy()
+        assert fg == pytest.approx(fo, rel=1e-8), it
+        assert eg == pytest.approx(eo, rel=1e-10), it
+
+
+def test_scft_variable_cell_config1_fp64():
+    # Config-1 system (64x1x1, N=100, fA=0.5, chi=0.2) from a lamellar start, the README's
+    # Anderson + DomainUpdater settings; plain FP64 mode must land on the oracle's result.
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, (64, 1, 1), 1, "lamellar", (10.0,), chi=0.2, **extra)
+
+    (so, co_cell, _), (sg, cg_cell, _) = both(build, cos_fields((1, 1, 64)))
+    opts = dict(nsteps=1500, ftol=1e-4, stol=1e-4)
+    ro = O.scft(so, O.AndersonUpdater(nhist=10), domain=O.DomainUpdater(nskip=5, nper=20),
+                opts=O.SCFTOptions(**opts))
+    rg = P.scft(sg, P.AndersonUpdater(nhist=10), domain=P.DomainUpdater(nskip=5, nper=20),
+                opts=P.SCFTOptions(**opts))
+    assert rg.converged and ro.converged
+    assert rg.nsteps == ro.nsteps
+    assert rg.fhelm == pytest.approx(ro.fhelm, rel=1e-9)
+    assert cg_cell.params[0] == pytest.approx(co_cell.params[0], rel=1e-9)
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 1e-9
+
+
+def test_readme_state_on_gpu():
+    sys, cell, ch = diblock_system(P, (64, 1, 1), 1, "lamellar", (16.528709413909947,), chi=0.2)
+    P.fieldinit(sys, fields=cos_fields(sys.shape))
+    res = P.scft(sys, P.AndersonUpdater(nhist=10), opts=P.SCFTOptions(nsteps=400, ftol=1e-10))
+    assert res.converged
+    assert res.fhelm == pytest.approx(0.0812381288285528, abs=5e-8)
+
+
+def test_hex_cylinder_notebook_state_on_gpu():
+    d, fields = load_seed("cyl2d")
+    sys, cell, ch = diblock_system(P, (64, 64, 1), 2, "hexagonal", (float(d["cell_params"][0]),),
+                                   f=0.3, chi=0.3)
+    P.fieldinit(sys, fields=fields)
+    res = P.scft(sys, P.AndersonUpdater(nhist=10), opts=P.SCFTOptions(nsteps=150, ftol=1e-7))
+    assert res.ferr < 2e-7
+    assert res.fhelm == pytest.approx(0.10659231121049459, abs=1e-7)
+
+
+@pytest.mark.parametrize("name,dims,dim,crystal,params,kw", CASES[:4], ids=[c[0] for c in CASES[:4]])
+def test_fp32_propagator_error_vs_fp64(name, dims, dim, crystal, params, kw):
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, dims, dim, crystal, params, Ns=40, **kw, **extra)
+
+    shape = (dims[2], dims[1], dims[0])
+    (so, _, co), (sg, _, cg) = both(build, rand_fields(shape), precision="fp32")
+    assert cg.Q == pytest.approx(co.Q, rel=2e-5)
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 2e-5
+    assert sg.free_energy() == pytest.approx(so.free_energy(), abs=2e-6)
+    np.testing.assert_allclose(sg.scfstress(), so.scfstress(), rtol=1e-3, atol=1e-8)
+
+
+@pytest.mark.parametrize("kind", ["PEC", "PECE", "MOM", "RMS", "ADAM", "NAG"])
+def test_euler_momentum_updaters_fp64(kind):
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, (32, 1, 1), 1, "lamellar", (10.0,), **extra)
+
+    (so, _, _), (sg, _, _) = both(build, rand_fields((1, 1, 32), scale=0.1))
+    if kind in ("PEC", "PECE"):
+        uo, ug = O.EulerUpdater(method=kind, lam=0.005), P.EulerUpdater(method=kind, lam=0.005)
+    else:
+        uo = O.MomentumUpdater(method=kind, lam=0.005)
+        ug = P.MomentumUpdater(method=kind, lam=0.005)
+    uo.setup(so)
+    ug.setup(sg)
+    for _ in range(10):
+        uo.step()
+        ug.step()
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-10)
+    for m in (1, 2):
+        assert rel(sg.fields[m], so.fields[m]) < 1e-10
+
+
+def test_errors_are_raised_not_hidden():
+    sys, cell, ch = diblock_system(P, (16, 1, 1), 1, "lamellar", (10.0,))
+    with pytest.raises(P.PolyFieldsError, match="initialized"):
+        sys.validate()
+    with pytest.raises(P.PolyFieldsError, match="descending"):
+        P.FieldSystem((1, 16, 1), P.UnitCell(1, "lamellar", 5.0))
