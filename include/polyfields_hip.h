@@ -135,6 +135,10 @@ int pf_updater_destroy(pf_updater* u);
 /* Variable cell */
 int pf_domain_create(pf_domain** out, int nper, int nskip, double lam, double pmax, double rlxn);
 int pf_domain_setup(pf_domain* d, pf_system* sys);
+/* Forward-difference step of the stress Jacobian; the reference fixes it at 1e-8
+   (updaters/domain.jl:101), which is the default.  Larger steps make the Jacobian
+   insensitive to last-bit rounding (the reference's step amplifies it ~1e8 times). */
+int pf_domain_set_fd_step(pf_domain* d, double dx);
 int pf_domain_step(pf_domain* d, int nsteps, double tol);
 int pf_domain_stress(pf_domain* d, int refresh, double* stress, int nparams, double* serr);
 int pf_domain_destroy(pf_domain* d);

@@ -148,9 +148,10 @@ class AndersonUpdater:
 class DomainUpdater:
     """src/updaters/domain.jl:7-120 (variable cell, forward-difference Jacobian)."""
 
-    def __init__(self, nper=10, nskip=200, lam=0.1, pmax=1.0, rlxn=0.5):
+    def __init__(self, nper=10, nskip=200, lam=0.1, pmax=1.0, rlxn=0.5, fd_step=1e-8):
         self.nper, self.nskip = int(nper), int(nskip)
         self.lam, self.pmax, self.rlxn = float(lam), float(pmax), float(rlxn)
+        self.fd_step = float(fd_step)   # 1e-8 in the reference (domain.jl:101)
         self.system = None
 
     def stress_error(self):
@@ -199,7 +200,7 @@ class DomainUpdater:
         sys = self.system
         cell = sys.cell
         self.oldparams = cell.params.copy()
-        dx = 1e-8
+        dx = self.fd_step
         x = 1.0 / dx
         for k in range(cell.nparams):
             cell.params = self.oldparams.copy()

@@ -324,6 +324,13 @@ int pf_domain_create(pf_domain** out, int nper, int nskip, double lam, double pm
     });
 }
 
+int pf_domain_set_fd_step(pf_domain* d, double dx) {
+    return guard([&] {
+        need(d && d->d && dx > 0.0, "finite-difference step");
+        d->d->fd_step = dx;
+    });
+}
+
 int pf_domain_setup(pf_domain* d, pf_system* sys) {
     return guard([&] {
         need(d && d->d, "domain");

@@ -177,6 +177,7 @@ struct Momentum : Updater {
 struct Domain {
     int nper, nskip;
     double lam, pmax, rlxn;
+    double fd_step = 1e-8;           // forward-difference step of dstress! (domain.jl:101)
     System* sys = nullptr;
     std::vector<double> oldparams, stepv, stress, dstress;
     Domain(int nper, int nskip, double lam, double pmax, double rlxn)

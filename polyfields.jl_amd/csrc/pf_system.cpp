@@ -590,7 +590,7 @@ void Domain::dstress_update() {
     Cell& cell = s.cell;
     const int np = (int)cell.params.size();
     oldparams = cell.params;
-    const double dx = 1e-8, x = 1.0 / dx;
+    const double dx = fd_step, x = 1.0 / dx;
     for (int k = 0; k < np; ++k) {
         cell.params = oldparams;
         cell.params[k] += dx;

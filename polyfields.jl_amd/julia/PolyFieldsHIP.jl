@@ -1,0 +1,231 @@
+"""
+    PolyFieldsHIP
+
+`ccall` binding of `libpolyfields_hip.so` (include/polyfields_hip.h) for PolyFields.jl.
+It mirrors the reference's user-facing API -- `FieldSystem`, `add_species!`,
+`add_interaction!`, `fieldinit!`, `scft!`, `AndersonUpdater`, `DomainUpdater`,
+`density!`, `residuals!`, `free_energy`, `field_error`, `scfstress` -- with every grid held
+on one MI355X.  Grids cross the boundary as `Array{Float64,3}` in Julia's own column-major
+(x fastest) order, so `pointer(A)` is passed without a transpose.
+
+Source-only in this repository: it is not executed by the test suite (no Julia toolchain
+in the build image); the Python ctypes mirror in `polyfields_hip/` exercises the same ABI.
+"""
+module PolyFieldsHIP
+
+using PolyFields: UnitCell, Monomer, Multiblock, Point, FloryInteraction, EdwardsInteraction,
+                  Compressibility, SCFTOptions, SCFTResults, Canonical, Grand
+
+const LIB = get(ENV, "POLYFIELDS_HIP_LIB", joinpath(@__DIR__, "..", "polyfields_hip", "libpolyfields_hip.so"))
+
+const PF_FP64 = Cint(0)
+const PF_FP32 = Cint(1)
+
+struct PFError <: Exception
+    code::Int
+    msg::String
+end
+
+function check(rc::Cint)
+    if rc < 0
+        msg = unsafe_string(ccall((:pf_last_error, LIB), Cstring, ()))
+        throw(PFError(rc, msg))
+    end
+    return nothing
+end
+
+mutable struct HIPSystem
+    handle::Ptr{Cvoid}
+    dims::NTuple{3,Int}
+    cell::UnitCell
+    mids::Vector{Int}
+    species::Vector{Any}
+end
+
+"""
+    FieldSystem(dims, cell; monomers, ensemble = Canonical, compress = Compressibility(0.0),
+                precision = :fp64, device = 0)
+
+Device-resident counterpart of `PolyFields.FieldSystem` (src/system/system.jl:51-74).
+"""
+function FieldSystem(dims::NTuple{3,<:Integer}, cell::UnitCell; monomers = Monomer[],
+                     ensemble = Canonical, compress = Compressibility(0.0), precision = :fp64,
+                     device::Integer = 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    p = Vector{Float64}(cell.params)
+    check(ccall((:pf_system_create, LIB), Cint,
+                (Ref{Ptr{Cvoid}}, Cint, Cint, Cint, Cint, Cstring, Ptr{Float64}, Cint, Cint,
+                 Float64, Cint, Cint),
+                h, dims[1], dims[2], dims[3], cell.dim, String(cell.crystal), p, length(p),
+                ensemble == Canonical ? 0 : 1, compress.inv_zeta,
+                precision == :fp32 ? PF_FP32 : PF_FP64, device))
+    sys = HIPSystem(h[], Tuple(Int.(dims)), cell, Int[], Any[])
+    finalizer(s -> ccall((:pf_system_destroy, LIB), Cint, (Ptr{Cvoid},), s.handle), sys)
+    for mon in monomers
+        add_monomer!(sys, mon)
+    end
+    return sys
+end
+
+function add_monomer!(sys::HIPSystem, mon::Monomer)
+    check(ccall((:pf_system_add_monomer, LIB), Cint, (Ptr{Cvoid}, Cint, Float64, Float64),
+                sys.handle, mon.id, mon.vol, mon.charge))
+    push!(sys.mids, mon.id)
+    return nothing
+end
+
+function add_species!(sys::HIPSystem, chain::Multiblock, val::Real)
+    mids = Cint[m for m in chain.mid_block]
+    Nb = Cint[n for n in chain.N_block]
+    bb = Float64[b for b in chain.b_block]
+    idx = Ref{Cint}(0)
+    nsu = Ref{Cint}(0)
+    check(ccall((:pf_system_add_multiblock, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Float64}, Cint, Float64, Ref{Cint},
+                 Ref{Cint}),
+                sys.handle, length(mids), mids, Nb, bb, chain.Ns, val, idx, nsu))
+    push!(sys.species, chain)
+    return nothing
+end
+
+function add_species!(sys::HIPSystem, pt::Point, val::Real)
+    idx = Ref{Cint}(0)
+    check(ccall((:pf_system_add_point, LIB), Cint, (Ptr{Cvoid}, Cint, Float64, Ref{Cint}),
+                sys.handle, pt.mids[1], val, idx))
+    push!(sys.species, pt)
+    return nothing
+end
+
+function add_interaction!(sys::HIPSystem, itx::FloryInteraction)
+    for ((a, b), chi) in itx.chis
+        check(ccall((:pf_system_set_flory, LIB), Cint, (Ptr{Cvoid}, Cint, Cint, Float64),
+                    sys.handle, a, b, chi))
+    end
+    return nothing
+end
+
+function add_interaction!(sys::HIPSystem, itx::EdwardsInteraction)
+    for (m, u) in itx.uvals
+        check(ccall((:pf_system_set_edwards, LIB), Cint, (Ptr{Cvoid}, Cint, Float64),
+                    sys.handle, m, u))
+    end
+    return nothing
+end
+
+function setfield!(sys::HIPSystem, mid::Integer, w::Array{Float64,3})
+    @assert size(w) == sys.dims
+    check(ccall((:pf_system_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
+                sys.handle, mid, w))
+end
+
+function getgrid(sys::HIPSystem, which::Integer, mid::Integer)
+    out = Array{Float64,3}(undef, sys.dims)
+    check(ccall((:pf_system_get_grid, LIB), Cint, (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}),
+                sys.handle, which, mid, out))
+    return out
+end
+
+fields(sys::HIPSystem, mid) = getgrid(sys, 0, mid)
+density(sys::HIPSystem, mid) = getgrid(sys, 1, mid)
+residuals(sys::HIPSystem, mid) = getgrid(sys, 2, mid)
+
+"""`fieldinit!(sys; seed, scale)` (src/fileio.jl:14-23), drawing with Julia's RNG."""
+function fieldinit!(sys::HIPSystem; seed::Integer = -1, scale::Real = 0.1)
+    if seed >= 0
+        Random.seed!(seed)
+    end
+    for mid in sort(sys.mids)
+        setfield!(sys, mid, scale * randn(sys.dims))
+    end
+    residuals!(sys)
+end
+
+density!(sys::HIPSystem) = check(ccall((:pf_density, LIB), Cint, (Ptr{Cvoid},), sys.handle))
+residuals!(sys::HIPSystem) = check(ccall((:pf_residuals, LIB), Cint, (Ptr{Cvoid},), sys.handle))
+validate(sys::HIPSystem) = check(ccall((:pf_validate, LIB), Cint, (Ptr{Cvoid},), sys.handle))
+
+for (jl, c) in ((:field_error, :pf_field_error), (:free_energy, :pf_free_energy),
+                (:free_energy_bulk, :pf_free_energy_bulk))
+    @eval function $jl(sys::HIPSystem)
+        v = Ref{Float64}(0.0)
+        check(ccall(($(QuoteNode(c)), LIB), Cint, (Ptr{Cvoid}, Ref{Float64}), sys.handle, v))
+        return v[]
+    end
+end
+
+function scfstress(sys::HIPSystem)
+    out = zeros(length(sys.cell.params))
+    check(ccall((:pf_scfstress, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint), sys.handle, out,
+                length(out)))
+    return out
+end
+
+mutable struct HIPUpdater
+    handle::Ptr{Cvoid}
+end
+
+function AndersonUpdater(; nhist::Integer = 50, iter_euler::Integer = 1, lam_euler::Real = 0.01)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pf_updater_anderson, LIB), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint, Float64), h,
+                nhist, iter_euler, lam_euler))
+    u = HIPUpdater(h[])
+    finalizer(x -> ccall((:pf_updater_destroy, LIB), Cint, (Ptr{Cvoid},), x.handle), u)
+    return u
+end
+
+setup!(u::HIPUpdater, sys::HIPSystem) =
+    check(ccall((:pf_updater_setup, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), u.handle, sys.handle))
+step!(u::HIPUpdater) = check(ccall((:pf_updater_step, LIB), Cint, (Ptr{Cvoid},), u.handle))
+
+mutable struct HIPDomain
+    handle::Ptr{Cvoid}
+end
+
+function DomainUpdater(; nper::Integer = 10, nskip::Integer = 200, lam::Real = 0.1,
+                       pmax::Real = 1.0, rlxn::Real = 0.5)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pf_domain_create, LIB), Cint,
+                (Ref{Ptr{Cvoid}}, Cint, Cint, Float64, Float64, Float64), h, nper, nskip, lam,
+                pmax, rlxn))
+    d = HIPDomain(h[])
+    finalizer(x -> ccall((:pf_domain_destroy, LIB), Cint, (Ptr{Cvoid},), x.handle), d)
+    return d
+end
+
+struct PFOptions
+    nsteps::Cint
+    nsout::Cint
+    nsave::Cint
+    ftol::Cdouble
+    stol::Cdouble
+end
+
+struct PFResults
+    time::Cdouble
+    nsteps::Cint
+    converged::Cint
+    ferr::Cdouble
+    serr::Cdouble
+    fhelm::Cdouble
+end
+
+"""
+    scft!(sys, updater; domain = nothing, opts = SCFTOptions()) -> SCFTResults
+
+`scft!` of src/simulate.jl:60-148; the whole loop runs inside the library.
+"""
+function scft!(sys::HIPSystem, updater::HIPUpdater; domain = nothing,
+               opts::SCFTOptions = SCFTOptions())
+    o = Ref(PFOptions(opts.nsteps, opts.nsout, opts.nsave, opts.ftol, opts.stol))
+    r = Ref(PFResults(0, 0, 0, 0, 0, 0))
+    dh = domain === nothing ? C_NULL : domain.handle
+    check(ccall((:pf_scft, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{PFOptions}, Ref{PFResults}),
+                sys.handle, updater.handle, dh, o, r))
+    x = r[]
+    return SCFTResults(x.time, x.nsteps, x.converged != 0, x.ferr, x.serr, x.fhelm)
+end
+
+using Random
+
+end # module

