@@ -1,14 +1,23 @@
-"""Benchmark: 64^3 MDE contour steps/s of the SCFT propagator on one MI355X.
+"""Benchmark of the SCFT hot path on one MI355X.
 
-Workload (the cubic diblock of the headline metric): 64^3 grid, cubic cell
+Headline: 64^3 MDE contour steps/s.  Workload: 64^3 grid, cubic cell
 a = 4.0 Rg = 4.0*sqrt(100/6) b, AB diblock N = 100, fA = 0.35, bA = bB = 1, Ns = 100
-(the reference's chain_grid adjusts it to 102, so one density evaluation is 204 MDE
-steps: 102 for q and 102 for q+), chi N = 20.  Fields are 0.1*N(0,1) from numpy's
-default_rng(seed) plus a BCC {110} cosine bias of amplitude 1.0.  Propagator in FP32,
-fields, densities and sums in FP64.
+(the reference's chain_grid adjusts it to 102, so one density evaluation is 204 MDE steps:
+102 for q and 102 for q+), chi N = 20.  Fields are 0.1*N(0,1) from numpy's
+default_rng(seed + instance) plus a BCC {110} cosine bias of amplitude 1.0.  FP32
+propagator, FP64 fields and sums.  `--instances` independent systems are in flight (one
+batch: every propagator launch covers all of their lanes).  A "step" is one density!
+evaluation of every instance; value = MDE steps/s over the whole job.
 
-A "step" is one density!(sys) evaluation of every instance in flight (204 MDE steps
-each, plus Q and the Simpson densities).  value = MDE steps/s over the whole job.
+Secondary (`batched_scft`): SCFT solves/s of the batched sweep workload -- 2-D hexagonal
+diblocks with grids drawn from {32^2, 48^2, 64^2, 96^2}, Ns in [64, 128], integer N_A in
+[25, 45], chi N in [12, 30], a in [3, 6] Rg, fields 0.1*N(0,1) -- batched in FP32 versus
+the plain FP64 path solving the same instances one at a time.  Protocol for both paths:
+scft!(AndersonUpdater(nhist=10, iter_euler=10^9, lam_euler=0.1); ftol=1e-2) (pure Euler
+mixing) then scft!(AndersonUpdater(nhist=10); ftol=1e-6); a solve is an instance reaching
+ferr < 1e-6.  The batched path runs the Anderson phase in FP32 to ferr < 1e-5, then
+switches the propagator to FP64 and finishes to 1e-6 (FP32 residual noise sits near
+1e-6).
 """
 import argparse
 import json
@@ -17,6 +26,7 @@ import os
 import statistics
 import sys
 import time
+import warnings
 
 import numpy as np
 
@@ -49,7 +59,6 @@ def build(mod, precision=None, seed=0):
     kw = {} if precision is None else {"precision": precision}
     cell = mod.UnitCell(3, "cubic", A_CELL)
     a, b = mod.Monomer(1), mod.Monomer(2)
-    import warnings
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
         ch = mod.Diblock(a, b, N, FA, 1.0, 1.0, 100)
@@ -78,19 +87,110 @@ def cpu_baseline(seed):
         ts.append(time.perf_counter() - t0)
     t = statistics.median(ts)
     return {"value": 2 * ch.Ns / t, "unit": "MDE steps/s", "cores": cores, "kind": "port",
-            "sample": f"median of 3 FP64 density! evaluations of the 64^3 diblock "
+            "sample": f"median of 3 FP64 density! evaluations of one 64^3 instance "
                       f"({2 * ch.Ns} MDE steps each) after 1 warm-up; CPU oracle "
-                      f"(scipy.fft workers={cores})"}
+                      f"(scipy.fft workers={cores}, pointwise single-threaded)"}
+
+
+# ---------------------------------------------------------------------------------
+def sweep_instances(count, seed=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.choice([32, 48, 64, 96]))
+        Ns = int(rng.integers(64, 129))
+        NA = int(rng.integers(25, 46))
+        chiN = float(rng.uniform(12, 30))
+        a = float(rng.uniform(3, 6)) * math.sqrt(N / 6.0)
+        out.append((n, Ns, NA, chiN, a, 1000 + k))
+    return out
+
+
+def sweep_system(P, c, precision):
+    n, Ns, NA, chiN, a, fs = c
+    cell = P.UnitCell(2, "hexagonal", a)
+    A, B = P.Monomer(1), P.Monomer(2)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ch = P.Multiblock([A, B], [NA, N - NA], [1.0, 1.0], Ns)
+    itx = P.FloryInteraction()
+    itx.set_interaction(1, 2, chiN / N)
+    s = P.FieldSystem((n, n, 1), cell, monomers=[A, B], precision=precision)
+    s.add_species(ch, 1.0)
+    s.add_interaction(itx)
+    rng = np.random.default_rng(fs)
+    for m in (1, 2):
+        s.fields[m] = 0.1 * rng.standard_normal((1, n, n))
+    return s
+
+
+PH1 = dict(nhist=10, iter_euler=10 ** 9, lam_euler=0.1, nsteps=1000, ftol=1e-2)
+PH2 = dict(nhist=10, iter_euler=1, lam_euler=0.01, nsteps=1000, ftol=1e-6)
+PH2_FP32 = dict(PH2, ftol=1e-5)
+
+
+def batched_scft(P, count, plain_sample):
+    cfg = sweep_instances(count)
+    systems = [sweep_system(P, c, "fp32") for c in cfg]
+    b = P.Batch(systems)
+    t0 = time.perf_counter()
+    r1 = b.scft(**PH1)
+    mask = [int(r.converged) for r in r1]
+    r15 = b.scft(mask=mask, **PH2_FP32)
+    mask = [int(r.converged) for r in r15]
+    b.set_precision("fp64")                  # FP64 polish below the FP32 noise floor
+    r2 = b.scft(mask=mask, **PH2)
+    t_batch = time.perf_counter() - t0
+    solved = [i for i, r in enumerate(r2) if r.converged]
+    # plain FP64 path, one instance at a time, same protocol
+    t_plain, n_plain_solved, errs = 0.0, 0, []
+    for i, c in enumerate(cfg[:plain_sample]):
+        s = sweep_system(P, c, "fp64")
+        t0 = time.perf_counter()
+        ok = False
+        try:
+            q1 = P.scft(s, P.AndersonUpdater(nhist=10, iter_euler=10 ** 9, lam_euler=0.1),
+                        opts=P.SCFTOptions(nsteps=PH1["nsteps"], ftol=PH1["ftol"]))
+            if q1.converged:
+                q2 = P.scft(s, P.AndersonUpdater(nhist=10),
+                            opts=P.SCFTOptions(nsteps=PH2["nsteps"], ftol=PH2["ftol"]))
+                ok = q2.converged
+        except P.PolyFieldsError:
+            ok = False
+        t_plain += time.perf_counter() - t0
+        if ok:
+            n_plain_solved += 1
+            if r2[i].converged:
+                errs.append(abs(r2[i].fhelm - q2.fhelm))
+    batch_rate = len(solved) / t_batch
+    plain_rate = n_plain_solved / t_plain if t_plain > 0 else 0.0
+    return {
+        "instances": count, "batched_fp32_solved": len(solved), "batched_fp32_seconds": t_batch,
+        "batched_solves_per_s": batch_rate,
+        "plain_fp64_sample": plain_sample, "plain_fp64_solved": n_plain_solved,
+        "plain_fp64_seconds": t_plain, "plain_solves_per_s": plain_rate,
+        "speedup_vs_plain": batch_rate / plain_rate if plain_rate > 0 else None,
+        "fhelm_abs_err_fp32_vs_fp64": {"max": max(errs) if errs else None,
+                                       "n": len(errs)},
+        "protocol": "scft!(Anderson iter_euler=1e9 lam_euler=0.1, ftol 1e-2) then "
+                    "scft!(Anderson nhist=10, ftol 1e-6); the batched path runs the second "
+                    "phase in FP32 to ftol 1e-5 and polishes in FP64 to 1e-6; solved = "
+                    "ferr < 1e-6 in FP64",
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--instances", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-instances", type=int, default=256)
+    ap.add_argument("--plain-sample", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT backend runs on a single MI355X "
@@ -99,39 +199,41 @@ def main():
 
     import polyfields_hip as P
 
-    sg, ch = build(P, args.precision, args.seed)
-    P.fieldinit(sg, fields=seeded_fields(sg.shape, args.seed))
+    systems = []
+    for k in range(args.instances):
+        s, ch = build(P, args.precision, args.seed + k)
+        for m, w in seeded_fields(s.shape, args.seed + k).items():
+            s.fields[m] = w
+        systems.append(s)
+    batch = P.Batch(systems)
     nsteps_mde = 2 * ch.Ns
     for _ in range(args.warmup):
-        sg.density_update()
-    sg.synchronize()
+        batch.density_update()
 
-    sg.timing(True)
+    batch.timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        sg.density_update()
-    sg.synchronize()
+        batch.density_update()
     dt = time.perf_counter() - t0
-    slab_ms, slab_n = sg.timing_read(0)
-    pen_ms, pen_n = sg.timing_read(1)
-    sg.timing(False)
+    slab_ms, slab_n = batch.timing_read(0)
+    pen_ms, pen_n = batch.timing_read(1)
+    batch.timing(False)
 
     ms_per_step = dt * 1e3 / args.steps
-    value = nsteps_mde * args.steps / dt
+    value = nsteps_mde * args.instances * args.steps / dt
     s = 4 if args.precision == "fp32" else 8
     n = GRID ** 3
     nk = (GRID // 2 + 1) * GRID * GRID
-    lanes = 2
-    # algorithmic bytes per launch: slab pass reads the half spectrum (2nk reals) and LW (n),
-    # writes q (n) and the next half spectrum (2nk); the pencil pass reads and writes 2nk.
+    lanes = 2 * args.instances
+    # algorithmic bytes per launch, all lanes: the slab pass reads the half spectrum
+    # (2nk reals) and LW (n), writes q_s (n) and the next half spectrum (2nk); the pencil
+    # pass reads and writes the half spectrum (4nk).
     slab_bytes = s * (2 * n + 4 * nk) * lanes
     pen_bytes = s * (4 * nk) * lanes
     slab_avg = slab_ms / max(slab_n, 1) * 1e-3
     pen_avg = pen_ms / max(pen_n, 1) * 1e-3
-    if slab_ms >= pen_ms:
-        dom = ("xy-slab pass", slab_bytes, slab_avg)
-    else:
-        dom = ("z-pencil pass", pen_bytes, pen_avg)
+    dom = ("xy-slab pass", slab_bytes, slab_avg) if slab_ms >= pen_ms else \
+        ("z-pencil pass", pen_bytes, pen_avg)
     achieved = dom[1] / dom[2] if dom[2] > 0 else 0.0
     b_step = s * (3 * n + 4 * nk)
     line = {
@@ -147,16 +249,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if args.precision == "fp32" else "f64",
         "data": "synthetic (seeded 0.1*N(0,1) + BCC cosine bias)",
-        "config": {"workload": "64^3 cubic AB diblock N=100 fA=0.35 chiN=20 Ns=102, "
-                               "one density! (204 MDE steps) per step",
-                   "instances": 1, "lanes_in_flight": lanes, "precision": args.precision},
+        "config": {"workload": "64^3 cubic AB diblock N=100 fA=0.35 chiN=20 Ns=102; one "
+                               "density! (204 MDE steps) of every instance per step",
+                   "instances": args.instances, "lanes_in_flight": lanes,
+                   "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None,
-                     "bytes_per_launch": dom[1], "avg_launch_us": dom[2] * 1e6},
+                     "traffic": None, "bytes_per_launch": dom[1],
+                     "avg_launch_us": dom[2] * 1e6},
         "step_roofline_frac": value * b_step / HBM_PEAK,
         "kernel_avg_us": {"slab": slab_avg * 1e6, "pencil": pen_avg * 1e6},
     }
+    del batch, systems
+    if not args.no_sweep:
+        line["batched_scft"] = batched_scft(P, args.sweep_instances, args.plain_sample)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.seed)
     print(json.dumps(line))

@@ -114,6 +114,10 @@ int pf_system_species_scalars(pf_system* sys, int species, double* Q, double* ph
 /* set != 0: write params (length nparams) and update!(cell); else read them. */
 int pf_system_cell(pf_system* sys, double* params, int nparams, int set);
 int pf_system_synchronize(pf_system* sys);
+/* Switch the propagator precision (PF_PRECISION_FP64 / _FP32).  Fields, densities and
+   every sum are float64 in both modes, so the state carries over; the q stacks are
+   reallocated.  Used to polish an FP32-converged state in FP64. */
+int pf_system_set_precision(pf_system* sys, int precision);
 
 int pf_validate(pf_system* sys);
 int pf_density(pf_system* sys);
@@ -146,6 +150,30 @@ int pf_domain_destroy(pf_domain* d);
 /* Driver.  domain may be NULL (fixed cell). */
 int pf_scft(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
             pf_scft_results* res);
+
+/* Batches.  Systems of one precision on one device advance together: every propagator
+   launch covers the lanes of all systems (grouped by grid size), every system-level
+   kernel covers all systems.  The systems stay owned by the caller and must outlive the
+   batch; no other call may use them while a batch call runs.  pf_batch_scft runs
+   scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler)) with a fixed cell for every
+   system in lockstep, dropping each from the active set once ferr < ftol; `results`
+   receives one entry per system (time is the whole batch's wall time).  A system whose
+   field error turns NaN (scft! would raise "Diverging field updater") is dropped and
+   reported with ferr = NaN and converged = 0 (so is one whose Anderson matrix turns
+   singular); the others continue.  Systems with mask[i] == 0 are left untouched and
+   reported with ferr = NaN.
+   No counterpart in the reference: it extends scft! to many independent systems. */
+typedef struct pf_batch pf_batch;
+int pf_batch_create(pf_batch** out, pf_system* const* systems, int nsys);
+int pf_batch_destroy(pf_batch* b);
+int pf_batch_density(pf_batch* b);
+int pf_batch_set_precision(pf_batch* b, int precision);
+int pf_batch_residuals(pf_batch* b);
+int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
+                  double ftol, const int* mask /* nsys flags or NULL: all */,
+                  pf_scft_results* results);
+int pf_batch_timing(pf_batch* b, int on);
+int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches);
 
 /* Measurement: per-launch HIP-event timing of the propagator kernels on the system's
    stream.  kind: 0 xy-slab pass, 1 z-pencil pass, 2 whole-plane (1-D/2-D) sweep. */

@@ -2,7 +2,7 @@
 #include <cstring>
 
 #include "../../include/polyfields_hip.h"
-#include "pf_host.h"
+#include "pf_batch.h"
 
 struct pf_system {
     pf::System* s;
@@ -12,6 +12,10 @@ struct pf_updater {
 };
 struct pf_domain {
     pf::Domain* d;
+};
+struct pf_batch {
+    pf::Batch* b;
+    std::vector<int> all;
 };
 
 namespace {
@@ -229,6 +233,10 @@ int pf_system_synchronize(pf_system* sys) {
     return guard([&] { S(sys).sync(); });
 }
 
+int pf_system_set_precision(pf_system* sys, int precision) {
+    return guard([&] { S(sys).set_precision(precision); });
+}
+
 int pf_validate(pf_system* sys) {
     return guard([&] { S(sys).validate(); });
 }
@@ -377,6 +385,84 @@ int pf_scft(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* 
         res->ferr = r.ferr;
         res->serr = r.serr;
         res->fhelm = r.fhelm;
+    });
+}
+
+int pf_batch_create(pf_batch** out, pf_system* const* systems, int nsys) {
+    return guard([&] {
+        need(out && systems && nsys > 0, "batch systems");
+        std::vector<pf::System*> v;
+        for (int i = 0; i < nsys; ++i) v.push_back(&S(systems[i]));
+        auto* h = new pf_batch{nullptr, {}};
+        try {
+            h->b = new pf::Batch(v);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        for (int i = 0; i < nsys; ++i) h->all.push_back(i);
+        *out = h;
+    });
+}
+
+int pf_batch_destroy(pf_batch* b) {
+    return guard([&] {
+        if (!b) return;
+        delete b->b;
+        delete b;
+    });
+}
+
+static pf::Batch& B(pf_batch* b) {
+    need(b && b->b, "batch handle");
+    return *b->b;
+}
+
+int pf_batch_density(pf_batch* b) {
+    return guard([&] {
+        for (auto* s : B(b).sys) s->sync();
+        B(b).density(b->all);
+        PF_HIP(hipStreamSynchronize(B(b).stream));
+    });
+}
+
+int pf_batch_set_precision(pf_batch* b, int precision) {
+    return guard([&] { B(b).set_precision(precision); });
+}
+
+int pf_batch_residuals(pf_batch* b) {
+    return guard([&] {
+        for (auto* s : B(b).sys) s->sync();
+        B(b).residuals(b->all);
+        PF_HIP(hipStreamSynchronize(B(b).stream));
+    });
+}
+
+int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
+                  double ftol, const int* mask, pf_scft_results* results) {
+    return guard([&] {
+        need(results, "results");
+        need(nhist >= 1 && nhist <= PF_MAXHIST, "nhist in 1..16");
+        auto r = B(b).scft_anderson(nhist, iter_euler, lam_euler, nsteps, ftol, mask);
+        for (size_t i = 0; i < r.size(); ++i) {
+            results[i].time = r[i].time;
+            results[i].nsteps = r[i].nsteps;
+            results[i].converged = r[i].converged;
+            results[i].ferr = r[i].ferr;
+            results[i].serr = 0.0;
+            results[i].fhelm = r[i].fhelm;
+        }
+    });
+}
+
+int pf_batch_timing(pf_batch* b, int on) {
+    return guard([&] { B(b).eng->timing(on); });
+}
+
+int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches) {
+    return guard([&] {
+        need(total_ms, "output");
+        *total_ms = B(b).eng->timing_ms(kind, launches);
     });
 }
 

@@ -1,0 +1,278 @@
+// Batched SCFT: many independent FieldSystems advance in lockstep through the
+// scft!(sys, AndersonUpdater) loop of src/simulate.jl:60-148 (fixed cell).  Each
+// iteration is one set of launches for all active systems; the host solves each
+// system's small Anderson least-squares problem (src/updaters/anderson.jl:184-189) and
+// drops systems from the active set as they converge.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+
+#include "pf_batch.h"
+
+namespace pf {
+
+Batch::Batch(const std::vector<System*>& s) : sys(s) {
+    if (sys.empty()) throw Error(-1, "empty batch");
+    precision = sys[0]->precision;
+    for (System* x : sys) {
+        if (x->precision != precision) throw Error(-2, "all systems of a batch need one precision");
+        if (x->device != sys[0]->device) throw Error(-2, "all systems of a batch need one device");
+        if (x->nmon() > PF_MAXMON) throw Error(-2, "too many monomers");
+    }
+    PF_HIP(hipSetDevice(sys[0]->device));
+    PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const size_t ns = sys.size();
+    sums = (double*)dev_alloc(sizeof(double) * 256 * ns);
+    PF_HIP(hipHostMalloc((void**)&hsums, sizeof(double) * 256 * ns, hipHostMallocDefault));
+    d_step = (StepTask*)dev_alloc(sizeof(StepTask) * ns);
+    PF_HIP(hipHostMalloc((void**)&h_step, sizeof(StepTask) * ns, hipHostMallocDefault));
+    eng = make_batch_engine(*this);
+}
+
+Batch::~Batch() {
+    (void)hipSetDevice(sys[0]->device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    eng.reset();
+    dev_free(sums);
+    dev_free(d_step);
+    dev_free(d_act);
+    if (hsums) (void)hipHostFree(hsums);
+    if (h_step) (void)hipHostFree(h_step);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Batch::set_precision(int p) {
+    if (p == precision) return;
+    PF_HIP(hipStreamSynchronize(stream));
+    eng.reset();
+    for (System* s : sys) s->set_precision(p);
+    precision = p;
+    eng = make_batch_engine(*this);
+}
+
+void Batch::bind_active(const std::vector<int>& act) {
+    if (act == act_cache && d_act) return;
+    std::vector<SysTask> t;
+    for (int i : act) {
+        System& s = *sys[i];
+        SysTask k{};
+        k.ra = s.residual_args();
+        k.pool.nmon = s.nmon();
+        k.pool.dst = s.rho;
+        for (size_t j = 0; j < s.species.size(); ++j) {
+            const Species& sp = s.species[j];
+            for (size_t q = 0; q < sp.slots.size(); ++q) {
+                const int m = sp.slots[q];
+                if (k.pool.nterm[m] >= 8) throw Error(-2, "too many species share one monomer type");
+                k.pool.src[m][k.pool.nterm[m]] = sp.rho_sp + q * s.n;
+                k.pool.isp[m][k.pool.nterm[m]] = (int)j;
+                k.pool.nterm[m]++;
+            }
+        }
+        k.n = s.n;
+        k.nparts = reduce_parts(s.n);
+        k.gparts = std::min(reduce_parts(s.n * s.nmon()), (int)(s.partial_len / (PF_MAXHIST * (PF_MAXHIST + 3) / 2)));
+        k.nsp = (int)s.species.size();
+        k.sc = s.sc;
+        k.partial = s.partial;
+        k.gpartial = s.partial;
+        k.sums = sums + (size_t)i * 256;
+        t.push_back(k);
+    }
+    dev_free(d_act);
+    d_act = (SysTask*)dev_alloc(sizeof(SysTask) * t.size());
+    PF_HIP(hipMemcpyAsync(d_act, t.data(), sizeof(SysTask) * t.size(), hipMemcpyHostToDevice, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    tasks = t;
+    act_cache = act;
+}
+
+static int max_parts(const std::vector<SysTask>& t, bool gram) {
+    int m = 1;
+    for (auto& k : t) m = std::max(m, gram ? k.gparts : k.nparts);
+    return m;
+}
+
+void Batch::density(const std::vector<int>& act) {
+    bind_active(act);
+    eng->density(act);
+    size_t nmax = 0;
+    for (int i : act) nmax = std::max(nmax, sys[i]->n);
+    launch_muphi_pool_b(d_act, (int)act.size(), nmax, stream);
+}
+
+void Batch::residuals(const std::vector<int>& act) {
+    density(act);
+    size_t nmax = 0;
+    for (int i : act) nmax = std::max(nmax, sys[i]->n);
+    launch_residuals_b(d_act, (int)act.size(), max_parts(tasks, false), nmax, stream);
+}
+
+void Batch::sys_sums(const std::vector<int>& act, std::vector<double>& out) {
+    bind_active(act);
+    launch_sys_reduce_b(d_act, (int)act.size(), max_parts(tasks, false), stream);
+    PF_HIP(hipMemcpyAsync(hsums, sums, sizeof(double) * 256 * sys.size(), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    out.assign(sys.size() * 5, 0.0);
+    for (int i : act)
+        for (int k = 0; k < 5; ++k) out[(size_t)i * 5 + k] = hsums[(size_t)i * 256 + 32 + k];
+}
+
+static double ferr_of(const double* v) {
+    return std::sqrt(v[0]) / std::sqrt(2.220446049250313e-16 + v[1]);
+}
+
+static double fhelm_of(const System& s, const double* v) {
+    double ftrans = 0.0;
+    for (size_t i = 0; i < s.species.size(); ++i)
+        if (s.phi[i] > 1e-8) ftrans += s.phi[i] * (s.phi[i] - 1.0) / s.species[i].Nref;
+    return ftrans + (-v[2] / (double)s.n) + (v[3] / (double)s.n + v[4] / (double)s.n);
+}
+
+std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double lam_euler,
+                                              int nsteps, double ftol, const int* mask) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t ns = sys.size();
+    std::vector<int> act;
+    for (size_t i = 0; i < ns; ++i)
+        if (!mask || mask[i]) act.push_back((int)i);
+    std::vector<BatchResult> res(ns);
+    for (auto& r : res) r = {0, 0, std::nan(""), std::nan(""), 0.0, 0};
+    if (act.empty()) return res;
+    for (int i : act) {
+        System* s = sys[i];
+        if (!s->canonical) throw Error(-2, "Anderson mixing can only be used for systems in the canonical ensemble.");
+        s->validate();
+        s->sync();
+    }
+    const std::vector<int> run = act;
+    residuals(act);
+    and_state.clear();
+    for (size_t i = 0; i < ns; ++i) {
+        and_state.emplace_back(new Anderson(nhist, iter_euler, lam_euler));
+        if (std::find(run.begin(), run.end(), (int)i) == run.end()) continue;
+        Anderson& A = *and_state.back();
+        A.sys = sys[i];
+        const size_t N = sys[i]->n * sys[i]->nmon();
+        A.hist_r = (double*)dev_alloc(sizeof(double) * N * (nhist + 1));
+        A.hist_w = (double*)dev_alloc(sizeof(double) * N * (nhist + 1));
+        PF_HIP(hipMemcpyAsync(A.wslot(0), sys[i]->w, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+        A.is_setup = true;
+    }
+    std::vector<double> v;
+    sys_sums(act, v);
+    std::vector<int> next;
+    for (int i : act) {
+        const double fe = ferr_of(&v[(size_t)i * 5]);
+        res[i] = {fe < ftol, 0, fe, fhelm_of(*sys[i], &v[(size_t)i * 5]), 0.0, 0};
+        if (!(fe < ftol)) next.push_back(i);
+    }
+    act = next;
+    int step = 0;
+    while (!act.empty() && step < nsteps) {
+        // bookkeeping shared by the lockstep systems
+        Anderson& A0 = *and_state[act[0]];
+        const int iter = A0.iter + 1;
+        int ih;
+        double lam;
+        if (iter < nhist + 1) {
+            ih = iter - 1;
+            lam = 1.0 - std::pow(0.9, (double)iter);
+        } else {
+            ih = nhist;
+            lam = 1.0;
+        }
+        for (int i : act) {
+            and_state[i]->iter = iter;
+            and_state[i]->iter_hist = ih;
+            and_state[i]->lam = lam;
+        }
+        residuals(act);
+        const bool euler = iter <= and_state[act[0]]->iter_euler;
+        for (size_t k = 0; k < act.size(); ++k) {
+            Anderson& A = *and_state[act[k]];
+            StepTask& s = h_step[k];
+            std::memset(&s, 0, sizeof(StepTask));
+            s.h = euler ? 0 : ih;
+            s.euler = euler;
+            s.lam = lam;
+            s.lam_euler = A.lam_euler;
+            s.r_out = A.rslot(ih);
+            for (int nn = 0; nn < s.h; ++nn) {
+                s.fh.p[nn] = A.wslot(ih - (nn + 1));
+                s.rh.p[nn] = A.rslot(ih - (nn + 1));
+            }
+            s.f_iter = A.wslot(ih);
+            s.f_out = (iter < nhist + 1) ? A.wslot(iter) : A.wslot(0);
+        }
+        PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
+        if (!euler) {
+            launch_gram_b(ih, d_act, d_step, (int)act.size(), max_parts(tasks, true), stream);
+            PF_HIP(hipMemcpyAsync(hsums, sums, sizeof(double) * 256 * ns, hipMemcpyDeviceToHost, stream));
+            PF_HIP(hipStreamSynchronize(stream));
+            const int nu = ih * (ih + 1) / 2;
+            std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
+            for (size_t k = 0; k < act.size(); ++k) {
+                const double* g = hsums + (size_t)act[k] * 256 + 64;
+                int q = 0;
+                for (int m = 0; m < ih; ++m)
+                    for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
+                for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
+                if (!lu_inverse(ih, U.data(), inv.data())) {
+                    // scft! raises on a singular Gram matrix (anderson.jl:203-208); in a
+                    // batch the system is reported as failed and dropped
+                    res[act[k]].diverged = 1;
+                    for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = 0.0;
+                    continue;
+                }
+                matvec(ih, inv.data(), V.data(), C.data());
+                for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = C[nn];
+            }
+            PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
+        }
+        size_t Nmax = 0;
+        for (int i : act) Nmax = std::max(Nmax, sys[i]->n * sys[i]->nmon());
+        launch_mix_b(d_act, d_step, (int)act.size(), Nmax, stream);
+        if (iter >= nhist + 1)
+            for (int i : act) and_state[i]->base = (and_state[i]->base + 1) % (nhist + 1);
+        sys_sums(act, v);
+        next.clear();
+        for (int i : act) {
+            const double fe = ferr_of(&v[(size_t)i * 5]);
+            res[i].ferr = fe;
+            res[i].fhelm = fhelm_of(*sys[i], &v[(size_t)i * 5]);
+            res[i].nsteps = step;
+            if (res[i].diverged) {
+                res[i].ferr = std::nan("");
+            } else if (std::isnan(fe)) {
+                // scft! raises "Diverging field updater" (src/simulate.jl:109-111); in a
+                // batch the system is reported (ferr = NaN, not converged) and dropped
+                res[i].diverged = 1;
+            } else if (fe < ftol) {
+                res[i].converged = 1;
+            } else {
+                next.push_back(i);
+            }
+        }
+        act = next;
+        step += 1;
+    }
+    for (int i : act) res[i].nsteps = step;
+    // final state (src/simulate.jl:133-136)
+    const std::vector<int>& all = run;
+    residuals(all);
+    sys_sums(all, v);
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int i : all) {
+        if (!res[i].diverged) {
+            res[i].ferr = ferr_of(&v[(size_t)i * 5]);
+            res[i].fhelm = fhelm_of(*sys[i], &v[(size_t)i * 5]);
+        }
+        res[i].time = t;
+    }
+    PF_HIP(hipStreamSynchronize(stream));
+    return res;
+}
+
+}  // namespace pf

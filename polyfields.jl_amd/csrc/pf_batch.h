@@ -1,0 +1,103 @@
+// Batches of FieldSystems advanced together: every propagator launch covers the lanes
+// of all systems, and every system-level kernel covers all systems (blockIdx.y).
+#pragma once
+
+#include "pf_host.h"
+
+namespace pf {
+
+template <typename T>
+struct LwTask {
+    T* out;
+    const double* w;
+    double coef;
+    size_t n;
+};
+
+template <typename T>
+struct FillTask {
+    T* p;
+    size_t n;
+};
+
+// Static per-system description (uploaded when the batch membership changes).
+struct SysTask {
+    ResidualArgs ra;
+    PoolArgs pool;
+    size_t n;
+    int nparts;          // reduction workgroups for n points
+    int gparts;          // reduction workgroups for nmon*n points (Gram)
+    int nsp;
+    DevScalars* sc;
+    double* partial;     // reduction scratch of the system
+    double* gpartial;
+    double* sums;        // 256 doubles in the batch result table
+};
+
+// Per-iteration Anderson data of one system.
+struct StepTask {
+    HistPtrs fh, rh;     // logical history slots ih-1 .. ih-h (fields, residuals)
+    const double* f_iter;
+    double* f_out;       // history slot receiving the new field
+    double* r_out;       // history slot receiving the residual
+    double C[PF_MAXHIST];
+    double lam, lam_euler;
+    int h, euler;
+};
+
+template <typename T>
+void launch_lw_fill_b(const LwTask<T>* lw, int nlw, const FillTask<T>* fill, int nfill,
+                      size_t nmax, hipStream_t st);
+void launch_muphi_pool_b(const SysTask* d, int nsys, size_t nmax, hipStream_t st);
+void launch_residuals_b(const SysTask* d, int nsys, int maxparts, size_t nmax, hipStream_t st);
+void launch_sys_reduce_b(const SysTask* d, int nsys, int maxparts, hipStream_t st);
+void launch_gram_b(int h, const SysTask* d, const StepTask* s, int nsys, int maxparts,
+                   hipStream_t st);
+void launch_mix_b(const SysTask* d, const StepTask* s, int nsys, size_t Nmax, hipStream_t st);
+
+class BatchEngineBase {
+   public:
+    virtual ~BatchEngineBase() = default;
+    // density! of the systems listed in `active` (indices into the batch)
+    virtual void density(const std::vector<int>& active) = 0;
+    // HIP-event timing of the propagator launches; kind: 0 slab, 1 pencil, 2 planar
+    virtual void timing(int on) = 0;
+    virtual double timing_ms(int kind, int* count) = 0;
+};
+
+struct BatchResult {
+    int converged, nsteps;
+    double ferr, fhelm, time;
+    int diverged;
+};
+
+class Batch {
+   public:
+    std::vector<System*> sys;
+    hipStream_t stream = nullptr;
+    int precision = 0;
+    std::unique_ptr<BatchEngineBase> eng;
+    double* sums = nullptr;          // nsys * 256 device doubles
+    double* hsums = nullptr;         // pinned mirror
+    SysTask* d_sys = nullptr;        // all systems' descriptors
+    StepTask* d_step = nullptr;
+    StepTask* h_step = nullptr;      // pinned staging
+    std::vector<SysTask> tasks;
+    std::vector<int> act_cache;      // active set the compacted descriptors were built for
+    SysTask* d_act = nullptr;        // descriptors of the active systems, compacted
+    std::vector<std::unique_ptr<Anderson>> and_state;
+
+    Batch(const std::vector<System*>& s);
+    ~Batch();
+    void bind_active(const std::vector<int>& act);
+    void set_precision(int p);
+    void density(const std::vector<int>& act);
+    void residuals(const std::vector<int>& act);
+    void sys_sums(const std::vector<int>& act, std::vector<double>& out);  // 5 per system
+    std::vector<BatchResult> scft_anderson(int nhist, int iter_euler, double lam_euler,
+                                           int nsteps, double ftol, const int* mask);
+};
+
+std::unique_ptr<BatchEngineBase> make_batch_engine(Batch& b);
+
+}  // namespace pf

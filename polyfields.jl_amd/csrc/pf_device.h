@@ -1,0 +1,135 @@
+// Device helpers shared by the propagator kernels.
+#pragma once
+
+#include "pf_internal.h"
+
+namespace pf {
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
+// multiply by +i (INV) or -i (forward)
+template <typename T, bool INV>
+__device__ __forceinline__ cpx<T> mul_i(cpx<T> a) {
+    return INV ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+}
+
+template <typename T>
+__device__ __forceinline__ T exp_t(double a);
+template <>
+__device__ __forceinline__ double exp_t<double>(double a) { return exp(a); }
+template <>
+__device__ __forceinline__ float exp_t<float>(double a) { return expf((float)a); }
+
+// Small DFTs of length R in registers.  Forward uses exp(-2 pi i/R), inverse exp(+...).
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+    if constexpr (R == 2) {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const T c = T(-0.5);
+        const T s = T(0.86602540378443864676372317075294);
+        cpx<T> t1 = cadd(v[1], v[2]);
+        cpx<T> d = csub(v[1], v[2]);
+        cpx<T> m = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+        cpx<T> sd = mul_i<T, INV>(cpx<T>{s * d.x, s * d.y});
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, sd);
+        v[2] = csub(m, sd);
+    } else if constexpr (R == 4) {
+        cpx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+        cpx<T> b0 = cadd(v[1], v[3]), b1 = mul_i<T, INV>(csub(v[1], v[3]));
+        v[0] = cadd(a0, b0);
+        v[2] = csub(a0, b0);
+        v[1] = cadd(a1, b1);
+        v[3] = csub(a1, b1);
+    } else if constexpr (R == 8) {
+        // two length-4 DFTs on the even and odd samples, combined with w8^k
+        cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        dft<T, 4, INV>(e);
+        dft<T, 4, INV>(o);
+        const T h = T(0.70710678118654752440084436210485);
+        // w8^1 = h(1 -/+ i), w8^2 = -/+ i, w8^3 = h(-1 -/+ i)   (forward / inverse)
+        const cpx<T> o1 = INV ? cpx<T>{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)}
+                              : cpx<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+        const cpx<T> o2 = mul_i<T, INV>(o[2]);
+        const cpx<T> o3 = INV ? cpx<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)}
+                              : cpx<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+}
+
+__device__ __forceinline__ int fold(int i, int n) { return (2 * i <= n) ? i : n - i; }
+
+__device__ __forceinline__ double ksq_at(const double* G, double jx, double jy, double jz) {
+    const double kx = G[0] * jx + G[1] * jy + G[2] * jz;
+    const double ky = G[3] * jx + G[4] * jy + G[5] * jz;
+    const double kz = G[6] * jx + G[7] * jy + G[8] * jz;
+    return kx * kx + ky * ky + kz * kz;
+}
+
+template <typename T>
+__device__ __forceinline__ int step_block(const Lane<T>* L, int t, int* s_in) {
+    if (L->dir > 0) {
+        const int s = 1 + t;
+        *s_in = s;
+        int b = 0;
+        while (b + 1 < L->nblk && L->begin[b + 1] <= s) ++b;
+        return b;
+    }
+    const int s = L->nsteps + 1 - t;
+    *s_in = s;
+    int b = L->nblk - 1;
+    while (b > 0 && L->begin[b] + 1 > s) --b;
+    return b;
+}
+
+__device__ __forceinline__ double dksq_at(const double* dGG, int np, int k, int dim, double j0,
+                                          double j1, double j2) {
+    const double j[3] = {j0, j1, j2};
+    double acc = 0.0;
+    for (int l = 0; l < dim; ++l)
+        for (int m = 0; m < dim; ++m) acc += j[l] * j[m] * dGG[(l * 3 + m) * np + k];
+    return acc;
+}
+
+template <int NV>
+__device__ void block_sum(double* v, double* out) {
+    // wavefront reduction then across waves through LDS; result in out[0..NV) of thread 0
+    __shared__ double red[PF_MAXPAR * 16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+        if (lane == 0) red[k * 16 + wave] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += red[k * 16 + w];
+            out[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace pf

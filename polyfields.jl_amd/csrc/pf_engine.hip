@@ -4,7 +4,7 @@
 #include <cmath>
 #include <cstring>
 
-#include "pf_host.h"
+#include "pf_batch.h"
 
 namespace pf {
 
@@ -71,6 +71,7 @@ class Engine : public EngineBase {
     double* spart = nullptr;
     // timing of the propagator kernels: 0 slab, 1 pencil, 2 planar
     bool tim = false;
+    bool fixed = true;               // use the compile-time specialised kernels when available
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evlog;
     size_t evnext = 0;
@@ -189,6 +190,8 @@ class Engine : public EngineBase {
             tk.isp = (int)i;
             tk.rho_sp = sp.rho_sp;
             tk.nmon = (int)sp.slots.size();
+            tk.n = n;
+            tk.sc = s.sc;
             tasks.push_back(tk);
         }
         nrows = row;
@@ -231,6 +234,21 @@ class Engine : public EngineBase {
     }
 
    public:
+    // --- access for the batch engine -------------------------------------------------
+    void prepare() {
+        if (S->dirty_species || sb.size() != S->species.size()) build();
+        for (auto& L : lanes) std::memcpy(L.G, S->cell.G, sizeof(L.G));
+    }
+    const std::vector<Lane<T>>& host_lanes() const { return lanes; }
+    const std::vector<SimpsonTask<T>>& host_tasks() const { return tasks; }
+    T* q_stack(int isp) const { return sb[isp].q; }
+    T* qc_stack(int isp) const { return sb[isp].qc; }
+    T* lw_table(int isp, int blk) const { return sb[isp].lw[blk]; }
+    int tiles() const { return ntiles; }
+    int tile_w() const { return tile; }
+    int plane_lds() const { return lds_plane; }
+    int pencil_lds() const { return lds_pencil; }
+
     explicit Engine(System& s) : S(&s) {}
     ~Engine() override {
         free_all();
@@ -278,23 +296,31 @@ class Engine : public EngineBase {
         const int nl = (int)lanes.size();
         if (nl) {
             if (planar) {
-                timed(2, [&] { launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st); });
+                timed(2, [&] {
+                    if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
+                        launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st);
+                });
             } else {
                 const int nz = s.dims[2], ny = s.dims[1];
-                timed(0, [&] { launch_slab<T>(d_lanes, nl, nz, SLAB_FWD, 0, lds_plane, 256, st); });
+                timed(0, [&] {
+                    if (!fixed || !fixed_slab<T>(d_lanes, nl, s.dims[0], ny, nz, SLAB_FWD, 0, st))
+                        launch_slab<T>(d_lanes, nl, nz, SLAB_FWD, 0, lds_plane, 256, st);
+                });
                 for (int t = 0; t < maxNs; ++t) {
                     timed(1, [&] {
-                        launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st);
+                        if (!fixed || !fixed_pencil<T>(d_lanes, nl, s.dims[0], ny, nz, ntiles, tile, t, st))
+                            launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st);
                     });
                     timed(0, [&] {
-                        launch_slab<T>(d_lanes, nl, nz, SLAB_INVFWD, t, lds_plane, 256, st);
+                        if (!fixed || !fixed_slab<T>(d_lanes, nl, s.dims[0], ny, nz, SLAB_INVFWD, t, st))
+                            launch_slab<T>(d_lanes, nl, nz, SLAB_INVFWD, t, lds_plane, 256, st);
                     });
                 }
             }
         }
         const int nparts = reduce_parts(n);
         if (!tasks.empty())
-            launch_species_Q<T>(d_tasks, (int)tasks.size(), n, s.partial, nparts, s.sc, st);
+            launch_species_Q<T>(d_tasks, (int)tasks.size(), n, s.partial, nparts, st);
         for (size_t i = 0; i < s.species.size(); ++i) {
             const Species& sp = s.species[i];
             if (sp.kind != 1) continue;
@@ -302,7 +328,7 @@ class Engine : public EngineBase {
             launch_point_density(s.w + (size_t)slot * n, s.mon_vol[slot], n, sp.rho_sp,
                                  s.partial, nparts, s.sc, (int)i, st);
         }
-        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), n, s.sc, st);
+        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), n, st);
     }
 
     void build_stress() {
@@ -406,6 +432,241 @@ class Engine : public EngineBase {
         for (size_t i = 0; i < s.n; ++i) host[i] = (double)tmp[i];
     }
 };
+
+// ---------------------------------------------------------------------------------
+// Batch: lanes of every active system in one set of launches.  Lanes are grouped by grid
+// size; planar groups of different sizes run concurrently on their own streams.
+template <typename T>
+class BatchEngine : public BatchEngineBase {
+    Batch* B;
+    struct Group {
+        int nx, ny, nz, off, count, maxNs, ntiles, tile, lds_plane, lds_pencil;
+        hipStream_t st = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Group> groups;
+    std::vector<Lane<T>> lanes;
+    Lane<T>* d_lanes = nullptr;
+    std::vector<SimpsonTask<T>> tasks;
+    SimpsonTask<T>* d_tasks = nullptr;
+    std::vector<LwTask<T>> lwt;
+    LwTask<T>* d_lwt = nullptr;
+    std::vector<FillTask<T>> fillt;
+    FillTask<T>* d_fillt = nullptr;
+    double* partial = nullptr;
+    size_t partial_len = 0;
+    std::vector<int> cached;
+    std::vector<long> cell_versions;
+    hipEvent_t fork = nullptr;
+    size_t nmax = 0;
+    bool tim = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evlog;
+    size_t evnext = 0;
+
+    hipEvent_t ev() {
+        if (evnext >= evpool.size()) {
+            hipEvent_t e;
+            PF_HIP(hipEventCreate(&e));
+            evpool.push_back(e);
+        }
+        return evpool[evnext++];
+    }
+    size_t ncalls[3] = {0, 0, 0};
+    // events bracket one launch in 8 of each kind (recording every launch would add a
+    // gap between dependent kernels); the sample still spans the whole timed region
+    template <typename F>
+    void timed(int cls, hipStream_t s, F&& f) {
+        if (!tim || (ncalls[cls]++ % 8) != 0) {
+            f();
+            return;
+        }
+        hipEvent_t a = ev(), b = ev();
+        PF_HIP(hipEventRecord(a, s));
+        f();
+        PF_HIP(hipEventRecord(b, s));
+        evlog.push_back({cls, {a, b}});
+    }
+
+    Engine<T>& E(int i) { return *static_cast<Engine<T>*>(B->sys[i]->eng.get()); }
+
+    template <typename V>
+    V* upload_vec(V* old, const std::vector<V>& v) {
+        dev_free(old);
+        if (v.empty()) return nullptr;
+        V* d = (V*)dev_alloc(sizeof(V) * v.size());
+        PF_HIP(hipMemcpyAsync(d, v.data(), sizeof(V) * v.size(), hipMemcpyHostToDevice, B->stream));
+        PF_HIP(hipStreamSynchronize(B->stream));
+        return d;
+    }
+
+    void rebuild(const std::vector<int>& act) {
+        std::vector<std::vector<Lane<T>>> by_size;
+        std::vector<Group> g;
+        tasks.clear();
+        lwt.clear();
+        fillt.clear();
+        nmax = 0;
+        for (int i : act) {
+            System& s = *B->sys[i];
+            Engine<T>& e = E(i);
+            e.prepare();
+            nmax = std::max(nmax, s.n);
+            int gi = -1;
+            for (size_t k = 0; k < g.size(); ++k)
+                if (g[k].nx == s.dims[0] && g[k].ny == s.dims[1] && g[k].nz == s.dims[2]) gi = (int)k;
+            if (gi < 0) {
+                Group ng{};
+                ng.nx = s.dims[0];
+                ng.ny = s.dims[1];
+                ng.nz = s.dims[2];
+                ng.ntiles = e.tiles();
+                ng.tile = e.tile_w();
+                ng.lds_plane = e.plane_lds();
+                ng.lds_pencil = e.pencil_lds();
+                g.push_back(ng);
+                by_size.emplace_back();
+                gi = (int)g.size() - 1;
+            }
+            for (const auto& L : e.host_lanes()) {
+                by_size[gi].push_back(L);
+                g[gi].maxNs = std::max(g[gi].maxNs, L.nsteps);
+            }
+            for (const auto& tk : e.host_tasks()) tasks.push_back(tk);
+            for (size_t k = 0; k < s.species.size(); ++k) {
+                const Species& sp = s.species[k];
+                if (sp.kind != 0)
+                    throw Error(-2, "point species are not supported inside a batch");
+                for (int b = 0; b < sp.nblk(); ++b) {
+                    const int slot = sp.block_slot[b];
+                    lwt.push_back({e.lw_table((int)k, b), s.w + (size_t)slot * s.n,
+                                   (-(sp.ds_block[b] / 2.0)) * s.mon_vol[slot], s.n});
+                }
+                fillt.push_back({e.q_stack((int)k), s.n});
+                if (e.qc_stack((int)k)) fillt.push_back({e.qc_stack((int)k) + (size_t)sp.Ns * s.n, s.n});
+            }
+        }
+        lanes.clear();
+        for (size_t k = 0; k < g.size(); ++k) {
+            g[k].off = (int)lanes.size();
+            g[k].count = (int)by_size[k].size();
+            lanes.insert(lanes.end(), by_size[k].begin(), by_size[k].end());
+            if (k < groups.size() && groups[k].st) {
+                g[k].st = groups[k].st;
+                g[k].done = groups[k].done;
+                groups[k].st = nullptr;
+            } else {
+                PF_HIP(hipStreamCreateWithFlags(&g[k].st, hipStreamNonBlocking));
+                PF_HIP(hipEventCreateWithFlags(&g[k].done, hipEventDisableTiming));
+            }
+        }
+        for (auto& old : groups)
+            if (old.st) {
+                (void)hipStreamDestroy(old.st);
+                (void)hipEventDestroy(old.done);
+            }
+        groups = g;
+        d_lanes = upload_vec(d_lanes, lanes);
+        d_tasks = upload_vec(d_tasks, tasks);
+        d_lwt = upload_vec(d_lwt, lwt);
+        d_fillt = upload_vec(d_fillt, fillt);
+        const size_t need = tasks.size() * (size_t)reduce_parts(nmax) + 1;
+        if (need > partial_len) {
+            dev_free(partial);
+            partial = (double*)dev_alloc(sizeof(double) * need);
+            partial_len = need;
+        }
+        cached = act;
+        cell_versions.clear();
+        for (int i : act) cell_versions.push_back(B->sys[i]->cell.version);
+    }
+
+   public:
+    explicit BatchEngine(Batch& b) : B(&b) { PF_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming)); }
+    ~BatchEngine() override {
+        dev_free(d_lanes);
+        dev_free(d_tasks);
+        dev_free(d_lwt);
+        dev_free(d_fillt);
+        dev_free(partial);
+        for (auto& gr : groups)
+            if (gr.st) {
+                (void)hipStreamDestroy(gr.st);
+                (void)hipEventDestroy(gr.done);
+            }
+        (void)hipEventDestroy(fork);
+        for (auto e : evpool) (void)hipEventDestroy(e);
+    }
+
+    void timing(int on) override {
+        tim = on != 0;
+        evlog.clear();
+        evnext = 0;
+        ncalls[0] = ncalls[1] = ncalls[2] = 0;
+    }
+
+    double timing_ms(int which, int* count) override {
+        PF_HIP(hipStreamSynchronize(B->stream));
+        double tot = 0.0;
+        int c = 0;
+        for (auto& e : evlog)
+            if (e.first == which) {
+                float ms = 0.f;
+                PF_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+                tot += ms;
+                ++c;
+            }
+        if (count) *count = c;
+        return tot;
+    }
+
+    void density(const std::vector<int>& act) override {
+        bool stale = act != cached;
+        for (size_t k = 0; !stale && k < act.size(); ++k)
+            stale = B->sys[act[k]]->cell.version != cell_versions[k] || B->sys[act[k]]->dirty_species;
+        if (stale) rebuild(act);
+        hipStream_t st = B->stream;
+        launch_lw_fill_b<T>(d_lwt, (int)lwt.size(), d_fillt, (int)fillt.size(), nmax, st);
+        PF_HIP(hipEventRecord(fork, st));
+        for (auto& gr : groups) {
+            PF_HIP(hipStreamWaitEvent(gr.st, fork, 0));
+            const Lane<T>* dl = d_lanes + gr.off;
+            if (gr.nz == 1) {
+                timed(2, gr.st, [&] {
+                    if (!fixed_planar<T>(dl, gr.count, gr.nx, gr.ny, 0, gr.maxNs, gr.st))
+                        launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.lds_plane, 256, gr.st);
+                });
+            } else {
+                timed(0, gr.st, [&] {
+                    if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_FWD, 0, gr.st))
+                        launch_slab<T>(dl, gr.count, gr.nz, SLAB_FWD, 0, gr.lds_plane, 256, gr.st);
+                });
+                for (int t = 0; t < gr.maxNs; ++t) {
+                    timed(1, gr.st, [&] {
+                        if (!fixed_pencil<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, gr.ntiles, gr.tile, t, gr.st))
+                            launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st);
+                    });
+                    timed(0, gr.st, [&] {
+                        if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_INVFWD, t, gr.st))
+                            launch_slab<T>(dl, gr.count, gr.nz, SLAB_INVFWD, t, gr.lds_plane, 256, gr.st);
+                    });
+                }
+            }
+            PF_HIP(hipEventRecord(gr.done, gr.st));
+            PF_HIP(hipStreamWaitEvent(st, gr.done, 0));
+        }
+        const int nparts = reduce_parts(nmax);
+        if (!tasks.empty()) {
+            launch_species_Q<T>(d_tasks, (int)tasks.size(), nmax, partial, nparts, st);
+            launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
+        }
+    }
+};
+
+std::unique_ptr<BatchEngineBase> make_batch_engine(Batch& b) {
+    if (b.precision == 0) return std::unique_ptr<BatchEngineBase>(new BatchEngine<double>(b));
+    return std::unique_ptr<BatchEngineBase>(new BatchEngine<float>(b));
+}
 
 std::unique_ptr<EngineBase> make_engine(System& s) {
     if (s.precision == 0) return std::unique_ptr<EngineBase>(new Engine<double>(s));

@@ -1,0 +1,395 @@
+// Propagator kernels specialised at compile time for the grid sizes of the production
+// workloads (64^3, 128^3, 32^3; 2-D planes 32^2..128^2; 1-D lines 32..256).  Same
+// arithmetic as the generic kernels of pf_fft.hip (same Stockham formulation, same split
+// steps, same order of the LW / LD / 1/n multiplies), with:
+//   * every length, stride and trip count a constant, so index math is shifts and
+//     multiplies and load / store loops are fully unrolled (all loads in flight at once);
+//   * twiddle tables copied to LDS once per workgroup instead of read from global memory
+//     in every pass;
+//   * radix-8 passes (64 = 8*8, 128 = 8*8*2, 96 = 8*4*3) to cut the number of LDS
+//     round trips and workgroup barriers.
+#include "pf_device.h"
+
+namespace pf {
+
+struct CPlan {
+    int npass;
+    int r[12];
+    int ns[12];
+};
+
+constexpr CPlan cplan(int n) {
+    CPlan p{};
+    int m = n, ns = 1;
+    while (m % 8 == 0) { p.r[p.npass] = 8; p.ns[p.npass++] = ns; ns *= 8; m /= 8; }
+    while (m % 4 == 0) { p.r[p.npass] = 4; p.ns[p.npass++] = ns; ns *= 4; m /= 4; }
+    while (m % 2 == 0) { p.r[p.npass] = 2; p.ns[p.npass++] = ns; ns *= 2; m /= 2; }
+    while (m % 3 == 0) { p.r[p.npass] = 3; p.ns[p.npass++] = ns; ns *= 3; m /= 3; }
+    return p;
+}
+
+// One Stockham pass (compile-time).  NL lines of length N, element e of line l at
+// buf[l*LS + e*ES]; NT threads; twiddle w_N^m = tw[m*TWS] (LDS).
+template <typename T, int N, int R, int NS, int NL, int ES, int LS, int NT, bool INV, int TWS>
+__device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __restrict__ tw) {
+    constexpr int NB = N / R;
+    constexpr int BPR = (R == 8) ? 2 : (R == 2 ? 8 : 4);
+    static_assert(NB <= BPR * NT, "transform too long for the workgroup");
+    constexpr int LPR0 = (BPR * NT) / NB;
+    constexpr int LPR = LPR0 < 1 ? 1 : (LPR0 > NL ? NL : LPR0);
+    constexpr int TWM = (N / (NS * R)) * TWS;
+#pragma unroll
+    for (int l0 = 0; l0 < NL; l0 += LPR) {
+        const int total = ((NL - l0) < LPR ? (NL - l0) : LPR) * NB;
+        cpx<T> v[BPR][R];
+#pragma unroll
+        for (int i = 0; i < BPR; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (i * NT < total && b < total) {
+                const int line = l0 + b / NB, j = b % NB;
+                const cpx<T>* src = buf + line * LS + j * ES;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = src[r * NB * ES];
+                if constexpr (NS > 1) {
+                    const int k = j % NS;
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        cpx<T> w = tw[k * r * TWM];
+                        if (INV) w.y = -w.y;
+                        v[i][r] = cmul(v[i][r], w);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < BPR; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (i * NT < total && b < total) {
+                const int line = l0 + b / NB, j = b % NB;
+                dft<T, R, INV>(v[i]);
+                const int d = (j / NS) * NS * R + (j % NS);
+                cpx<T>* dst = buf + line * LS + d * ES;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * NS * ES] = v[i][r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int N, int NL, int ES, int LS, int NT, bool INV, int TWS, int IP = 0>
+__device__ __forceinline__ void cfft(cpx<T>* buf, const cpx<T>* tw) {
+    constexpr CPlan P = cplan(N);
+    if constexpr (IP < P.npass) {
+        cpass<T, N, P.r[IP], P.ns[IP], NL, ES, LS, NT, INV, TWS>(buf, tw);
+        cfft<T, N, NL, ES, LS, NT, INV, TWS, IP + 1>(buf, tw);
+    }
+}
+
+// Split steps of the packed real-row transform (see pf_fft.hip for the formulas).
+template <typename T, int M, int NROWS, int LD, int NT>
+__device__ __forceinline__ void csplit_r2c(cpx<T>* S, const cpx<T>* twN) {
+    constexpr int HALF = M / 2 + 1;
+#pragma unroll
+    for (int i0 = 0; i0 < NROWS * HALF; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NROWS * HALF) {
+            const int y = i / HALF, k = i % HALF;
+            cpx<T>* row = S + y * LD;
+            const int km = (M - k) % M;
+            const cpx<T> zk = row[k], zm = row[km];
+            const cpx<T> e = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+            const cpx<T> d = {zk.x - zm.x, zk.y + zm.y};
+            const cpx<T> o = {T(0.5) * d.y, T(-0.5) * d.x};
+            const cpx<T> wo = cmul(twN[k], o);
+            row[k] = cadd(e, wo);
+            if (M - k != k) row[M - k] = cconj(csub(e, wo));
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int M, int NROWS, int LD, int NT>
+__device__ __forceinline__ void csplit_c2r(cpx<T>* S, const cpx<T>* twN) {
+    constexpr int HALF = M / 2 + 1;
+#pragma unroll
+    for (int i0 = 0; i0 < NROWS * HALF; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NROWS * HALF) {
+            const int y = i / HALF, k = i % HALF;
+            cpx<T>* row = S + y * LD;
+            cpx<T> xa = row[k], xb = row[M - k];
+            if (k == 0) {
+                xa.y = T(0);
+                xb.y = T(0);
+            }
+            const cpx<T> e = {T(0.5) * (xa.x + xb.x), T(0.5) * (xa.y - xb.y)};
+            const cpx<T> dd = {T(0.5) * (xa.x - xb.x), T(0.5) * (xa.y + xb.y)};
+            const cpx<T> o = cmul(dd, cconj(twN[k]));
+            row[k] = {e.x - o.y, e.y + o.x};
+            if (k != 0 && M - k != k) row[M - k] = {e.x + o.y, -e.y + o.x};
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int NX, int NY, int NT>
+__device__ __forceinline__ void cplane_forward(cpx<T>* S, const cpx<T>* twx, const cpx<T>* twy) {
+    constexpr int M = NX / 2, LD = M + 1;
+    cfft<T, M, NY, 1, LD, NT, false, 2>(S, twx);
+    csplit_r2c<T, M, NY, LD, NT>(S, twx);
+    if constexpr (NY > 1) cfft<T, NY, M + 1, LD, 1, NT, false, 1>(S, twy);
+}
+
+template <typename T, int NX, int NY, int NT>
+__device__ __forceinline__ void cplane_inverse(cpx<T>* S, const cpx<T>* twx, const cpx<T>* twy) {
+    constexpr int M = NX / 2, LD = M + 1;
+    if constexpr (NY > 1) cfft<T, NY, M + 1, LD, 1, NT, true, 1>(S, twy);
+    csplit_c2r<T, M, NY, LD, NT>(S, twx);
+    cfft<T, M, NY, 1, LD, NT, true, 2>(S, twx);
+}
+
+template <typename T, int COUNT, int NT>
+__device__ __forceinline__ void lds_copy_in(cpx<T>* dst, const cpx<T>* __restrict__ src) {
+#pragma unroll
+    for (int i0 = 0; i0 < COUNT; i0 += NT)
+        if (i0 + (int)threadIdx.x < COUNT) dst[i0 + threadIdx.x] = src[i0 + threadIdx.x];
+}
+
+// packed real plane (pairs, M per row, NY rows) <-> LDS rows of stride M+1
+template <typename T, int NX, int NY, int NT, typename F>
+__device__ __forceinline__ void for_pairs(F&& f) {
+    constexpr int M = NX / 2, NP = M * NY;
+#pragma unroll
+    for (int i0 = 0; i0 < NP; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NP) f(i, (i / M) * (M + 1) + (i % M));
+    }
+}
+
+// ---------------------------------------------------------------------------------
+template <typename T, int NX, int NY, int NT>
+__global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ lanes, int t0, int t1) {
+    constexpr int M = NX / 2, LD = M + 1, N = NX * NY, NKP = NY * LD;
+    __shared__ cpx<T> S[NKP];
+    __shared__ cpx<T> TWX[NX];
+    __shared__ cpx<T> TWY[NY];
+    const Lane<T>* L = lanes + blockIdx.x;
+    const int tend = min(t1, L->nsteps);
+    if (t0 >= tend) return;
+    lds_copy_in<T, NX, NT>(TWX, L->twx);
+    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    const T scale = T(2.0 / (double)N);
+    int s_in;
+    step_block(L, t0, &s_in);
+    {
+        const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * N);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) { S[o] = qin[i]; });
+    }
+    const double* G = L->G;
+    for (int t = t0; t < tend; ++t) {
+        const int b = step_block(L, t, &s_in);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b]);
+        __syncthreads();
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> w = lw[i];
+            S[o].x *= w.x;
+            S[o].y *= w.y;
+        });
+        __syncthreads();
+        cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+        const double mc = -L->ldc[b];
+#pragma unroll
+        for (int i0 = 0; i0 < NKP; i0 += NT) {
+            const int i = i0 + threadIdx.x;
+            if (i < NKP) {
+                const int y = i / LD, kx = i % LD;
+                const T f = exp_t<T>(mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0));
+                S[i].x *= f;
+                S[i].y *= f;
+            }
+        }
+        __syncthreads();
+        cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
+        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * N);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> w = lw[i];
+            const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+            S[o] = v;
+            qout[i] = v;
+        });
+    }
+}
+
+template <typename T, int NX, int NY, int MODE, int NT>
+__global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
+    constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
+    __shared__ cpx<T> S[KPLANE];
+    __shared__ cpx<T> TWX[NX];
+    __shared__ cpx<T> TWY[NY];
+    const int lane = blockIdx.x / nz;
+    const int z = blockIdx.x - lane * nz;
+    const Lane<T>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    const size_t n = (size_t)PLANE * nz;
+    cpx<T>* kb = L->kbuf + (size_t)z * KPLANE;
+    lds_copy_in<T, NX, NT>(TWX, L->twx);
+    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    int s_in;
+    int b = step_block(L, t, &s_in);
+    if constexpr (MODE == SLAB_INVFWD) {
+        lds_copy_in<T, KPLANE, NT>(S, kb);
+        __syncthreads();
+        cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
+        const T scale = T(2.0 / (double)n);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE);
+        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
+                                                 (size_t)z * PLANE);
+        const bool more = t + 1 < L->nsteps;
+        int s2;
+        const int b2 = more ? step_block(L, t + 1, &s2) : b;
+        const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * PLANE);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> w = lw[i];
+            const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+            qout[i] = v;
+            const cpx<T> w2 = lw2[i];
+            S[o] = {v.x * w2.x, v.y * w2.y};
+        });
+        if (!more) return;
+    } else {
+        const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
+                                                            (size_t)z * PLANE);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> q = qin[i], w = lw[i];
+            S[o] = {q.x * w.x, q.y * w.y};
+        });
+    }
+    __syncthreads();
+    cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+#pragma unroll
+    for (int i0 = 0; i0 < KPLANE; i0 += NT)
+        if (i0 + (int)threadIdx.x < KPLANE) kb[i0 + threadIdx.x] = S[i0 + threadIdx.x];
+}
+
+// z-pencil pass over KB consecutive ky rows and all NXH kx columns: in the half-spectrum
+// layout [kz][ky][kx] those KB*NXH values are contiguous for every kz, so each z row is
+// one coalesced run.
+template <typename T, int NZ, int NXH, int KB, int NT>
+__global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ lanes, int ny, int t) {
+    constexpr int C = KB * NXH;
+    __shared__ cpx<T> P[NZ * C];
+    __shared__ cpx<T> TWZ[NZ];
+    const int per = ny / KB;
+    const int lane = blockIdx.x / per;
+    const int ky0 = (blockIdx.x - lane * per) * KB;
+    const Lane<T>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    lds_copy_in<T, NZ, NT>(TWZ, L->twz);
+    int s_in;
+    const int b = step_block(L, t, &s_in);
+    cpx<T>* kb = L->kbuf + (size_t)ky0 * NXH;
+    const size_t zstride = (size_t)ny * NXH;
+#pragma unroll
+    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NZ * C) P[i] = kb[(i / C) * zstride + (i % C)];
+    }
+    __syncthreads();
+    cfft<T, NZ, C, C, 1, NT, false, 1>(P, TWZ);
+    const double mc = -L->ldc[b];
+    const double* G = L->G;
+#pragma unroll
+    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NZ * C) {
+            const int z = i / C, c = i % C;
+            const int ky = ky0 + c / NXH, kx = c % NXH;
+            const T f = exp_t<T>(mc * ksq_at(G, (double)kx, (double)fold(ky, ny), (double)fold(z, NZ)));
+            P[i].x *= f;
+            P[i].y *= f;
+        }
+    }
+    __syncthreads();
+    cfft<T, NZ, C, C, 1, NT, true, 1>(P, TWZ);
+#pragma unroll
+    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NZ * C) kb[(i / C) * zstride + (i % C)] = P[i];
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Dispatch: returns false when the size has no specialisation (generic kernels then).
+
+template <typename T>
+bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st) {
+#define PF_PL(X, Y, NT)                                                                    \
+    if (nx == X && ny == Y) {                                                              \
+        hipLaunchKernelGGL((k_planar_fx<T, X, Y, NT>), dim3(nl), dim3(NT), 0, st, d, t0, t1); \
+        PF_HIP(hipGetLastError());                                                         \
+        return true;                                                                       \
+    }
+    PF_PL(32, 1, 64)
+    PF_PL(64, 1, 64)
+    PF_PL(128, 1, 64)
+    PF_PL(256, 1, 128)
+    PF_PL(32, 32, 256)
+    PF_PL(48, 48, 256)
+    PF_PL(64, 64, 256)
+    PF_PL(96, 96, 256)
+    PF_PL(128, 128, 256)
+#undef PF_PL
+    return false;
+}
+
+template <typename T>
+bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
+                hipStream_t st) {
+#define PF_SL(X, Y)                                                                         \
+    if (nx == X && ny == Y) {                                                               \
+        if (mode == SLAB_FWD)                                                               \
+            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_FWD, 256>), dim3(nl * nz), dim3(256), 0, \
+                               st, d, nz, t);                                               \
+        else                                                                                \
+            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_INVFWD, 256>), dim3(nl * nz), dim3(256), \
+                               0, st, d, nz, t);                                            \
+        PF_HIP(hipGetLastError());                                                          \
+        return true;                                                                        \
+    }
+    PF_SL(32, 32)
+    PF_SL(64, 64)
+    if constexpr (sizeof(T) == 4) { PF_SL(128, 128) }
+#undef PF_SL
+    return false;
+}
+
+// ky rows per workgroup: 4, 2, 1 for nx = 32, 64, 128 (NXH = 17, 33, 65)
+template <typename T>
+bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, int t,
+                  hipStream_t st) {
+#define PF_PE(X, Z, KB)                                                                     \
+    if (nx == X && nz == Z && ny % KB == 0) {                                               \
+        hipLaunchKernelGGL((k_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(nl * (ny / KB)),    \
+                           dim3(256), 0, st, d, ny, t);                                     \
+        PF_HIP(hipGetLastError());                                                          \
+        return true;                                                                        \
+    }
+    PF_PE(32, 32, 4)
+    PF_PE(64, 64, 2)
+    if constexpr (sizeof(T) == 4) { PF_PE(128, 128, 1) }
+#undef PF_PE
+    return false;
+}
+
+template bool fixed_planar<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
+template bool fixed_planar<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
+template bool fixed_slab<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t);
+template bool fixed_slab<double>(const Lane<double>*, int, int, int, int, int, int, hipStream_t);
+template bool fixed_pencil<float>(const Lane<float>*, int, int, int, int, int, int, int, hipStream_t);
+template bool fixed_pencil<double>(const Lane<double>*, int, int, int, int, int, int, int,
+                                   hipStream_t);
+
+}  // namespace pf

@@ -114,6 +114,7 @@ struct System {
     std::vector<double> scfstress(int exact = 0);
     void monomer_fractions(std::vector<double>& fr);
     void sync();
+    void set_precision(int p);       // swap the propagator engine (fields stay FP64)
 };
 
 class EngineBase {

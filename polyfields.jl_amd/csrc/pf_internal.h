@@ -78,6 +78,8 @@ struct StressLane {
     cpx<T>* kqc;             // 3-D: nk scratch for q+_s
 };
 
+struct DevScalars;
+
 // Species entry for the Simpson density / Q kernels.
 template <typename T>
 struct SimpsonTask {
@@ -91,6 +93,8 @@ struct SimpsonTask {
     int isp;                 // species index (scalar slots)
     double* rho_sp;          // species density per monomer slot: rho_sp[slot*n + i]
     int nmon;
+    size_t n;                // grid points of the owning system
+    DevScalars* sc;          // scalar slots of the owning system
 };
 
 // System-level scalar slots kept on the device.
@@ -125,6 +129,16 @@ void launch_stress_pencil(const StressLane<T>* d_lanes, int nlanes, int ny, int 
                           hipStream_t st);
 
 enum SlabMode { SLAB_FWD = 0, SLAB_INVFWD = 1 };
+
+// compile-time specialised variants (pf_fft_fixed.hip); false: size not specialised
+template <typename T>
+bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st);
+template <typename T>
+bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
+                hipStream_t st);
+template <typename T>
+bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int ntiles, int tile, int t,
+                  hipStream_t st);
 
 // --- launchers (pf_kernels.hip) ---------------------------------------------------
 struct ResidualArgs {
@@ -166,11 +180,10 @@ void launch_lw_table(T* out, const double* w, double coef, size_t n, hipStream_t
 template <typename T>
 void launch_fill(T* p, T v, size_t n, hipStream_t st);
 template <typename T>
-void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t n, double* partial,
-                      int nparts, DevScalars* sc, hipStream_t st);
+void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, double* partial,
+                      int nparts, hipStream_t st);
 template <typename T>
-void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t n, DevScalars* sc,
-                    hipStream_t st);
+void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st);
 void launch_point_density(const double* w, double vol, size_t n, double* rho_sp,
                           double* partial, int nparts, DevScalars* sc, int isp,
                           hipStream_t st);

@@ -17,6 +17,11 @@ int reduce_parts(size_t n) {
     return (int)p;
 }
 
+__device__ __forceinline__ int reduce_parts_dev(size_t n) {
+    size_t p = (n + RB * 4 - 1) / (RB * 4);
+    return (int)(p < 1 ? 1 : (p > 512 ? 512 : p));
+}
+
 template <int NV>
 __device__ void wg_reduce_store(double* v, double* out_row) {
     __shared__ double red[NV > 0 ? NV * 4 : 1];
@@ -66,48 +71,59 @@ void launch_fill(T* p, T v, size_t n, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------------
-// Q = sum(q_{Ns+1}) / n  (src/species/multiblock.jl:175)
+// Q = sum(q_{Ns+1}) / n  (src/species/multiblock.jl:175); one row of partials per task.
 template <typename T>
 __global__ void __launch_bounds__(RB) k_sum_slice(const SimpsonTask<T>* __restrict__ tasks,
-                                                  size_t n, double* partial, int nparts) {
+                                                  double* partial, int nparts) {
     const SimpsonTask<T>* tk = tasks + blockIdx.y;
+    const size_t n = tk->n;
+    // each task keeps the partition of its own size, so a task sums identically whether
+    // it runs alone or inside a batch of larger systems
+    const int own = reduce_parts_dev(n);
+    if ((int)blockIdx.x >= own) return;
     const T* q = tk->q + (size_t)tk->Ns * n;
     double acc = 0.0;
-    for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)nparts * RB)
+    for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < n; i += (size_t)own * RB)
         acc += (double)q[i];
     double v[1] = {acc};
     wg_reduce_store<1>(v, partial + (size_t)blockIdx.y * nparts + blockIdx.x);
 }
 
 template <typename T>
-__global__ void k_finish_Q(const SimpsonTask<T>* __restrict__ tasks, const double* partial,
-                           int nparts, size_t n, DevScalars* sc) {
+__global__ void __launch_bounds__(64) k_finish_Q(const SimpsonTask<T>* __restrict__ tasks,
+                                                 const double* partial, int nparts) {
     const SimpsonTask<T>* tk = tasks + blockIdx.x;
+    // fixed-order two-level sum: lane l adds parts l, l+64, ...; then lane 0 adds lanes
+    __shared__ double part[64];
+    const int own = reduce_parts_dev(tk->n);
+    double s = 0.0;
+    for (int p = threadIdx.x; p < own; p += 64) s += partial[(size_t)blockIdx.x * nparts + p];
+    part[threadIdx.x] = s;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int p = 0; p < nparts; ++p) s += partial[(size_t)blockIdx.x * nparts + p];
-        sc->Q[tk->isp] = s / (double)n;
+        double t = 0.0;
+        for (int l = 0; l < 64; ++l) t += part[l];
+        tk->sc->Q[tk->isp] = t / (double)tk->n;
     }
 }
 
 template <typename T>
-void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t n, double* partial,
-                      int nparts, DevScalars* sc, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_slice<T>, dim3(nparts, ntasks), dim3(RB), 0, st, d_tasks, n,
-                       partial, nparts);
-    hipLaunchKernelGGL(k_finish_Q<T>, dim3(ntasks), dim3(64), 0, st, d_tasks, partial, nparts, n,
-                       sc);
+void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, double* partial,
+                      int nparts, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_slice<T>, dim3(nparts, ntasks), dim3(RB), 0, st, d_tasks, partial,
+                       nparts);
+    hipLaunchKernelGGL(k_finish_Q<T>, dim3(ntasks), dim3(64), 0, st, d_tasks, partial, nparts);
     PF_HIP(hipGetLastError());
 }
 
 // Simpson contour integration per block, normalisation by Q Nref, and block -> monomer
 // sums (src/species/multiblock.jl:182-228), in the reference's order of operations.
 template <typename T>
-__global__ void k_simpson(const SimpsonTask<T>* __restrict__ tasks, size_t n,
-                          const DevScalars* __restrict__ sc) {
+__global__ void k_simpson(const SimpsonTask<T>* __restrict__ tasks) {
     const SimpsonTask<T>* tk = tasks + blockIdx.y;
     const int Ns = tk->Ns;
-    const double QN = sc->Q[tk->isp] * tk->Nref;
+    const size_t n = tk->n;
+    const double QN = tk->sc->Q[tk->isp] * tk->Nref;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         double acc_m[PF_MAXMON];
@@ -132,10 +148,9 @@ __global__ void k_simpson(const SimpsonTask<T>* __restrict__ tasks, size_t n,
 }
 
 template <typename T>
-void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t n, DevScalars* sc,
-                    hipStream_t st) {
-    int g = (int)std::min<size_t>((n + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_simpson<T>, dim3(g, ntasks), dim3(256), 0, st, d_tasks, n, sc);
+void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st) {
+    int g = (int)std::min<size_t>((nmax + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_simpson<T>, dim3(g, ntasks), dim3(256), 0, st, d_tasks);
     PF_HIP(hipGetLastError());
 }
 
@@ -468,9 +483,8 @@ void launch_segsum(const double* in, int nseg, int nparts, int stride, int nvals
     template void launch_lw_table<T>(T*, const double*, double, size_t, hipStream_t);        \
     template void launch_fill<T>(T*, T, size_t, hipStream_t);                                \
     template void launch_species_Q<T>(const SimpsonTask<T>*, int, size_t, double*, int,     \
-                                      DevScalars*, hipStream_t);                            \
-    template void launch_simpson<T>(const SimpsonTask<T>*, int, size_t, DevScalars*,         \
-                                    hipStream_t);
+                                      hipStream_t);                                         \
+    template void launch_simpson<T>(const SimpsonTask<T>*, int, size_t, hipStream_t);
 PF_INST2(float)
 PF_INST2(double)
 

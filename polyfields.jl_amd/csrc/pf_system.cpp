@@ -98,6 +98,16 @@ int System::slot_of(int id) const {
 
 void System::sync() { PF_HIP(hipStreamSynchronize(stream)); }
 
+void System::set_precision(int p) {
+    if (p != 0 && p != 1) throw Error(-2, "precision must be 0 (fp64) or 1 (fp32)");
+    if (p == precision) return;
+    sync();
+    eng.reset();
+    precision = p;
+    eng = make_engine(*this);
+    dirty_species = 1;
+}
+
 void System::add_monomer(int id, double vol, double charge) {
     if (!species.empty()) throw Error(-2, "monomers must be added before species");
     if (vol <= 0.0) throw Error(-2, "monomer volume must be positive");

@@ -49,6 +49,7 @@ SIGNATURES = {
     "pf_system_species_scalars": (I, [P, I, DP, DP, DP]),
     "pf_system_cell": (I, [P, DP, I, I]),
     "pf_system_synchronize": (I, [P]),
+    "pf_system_set_precision": (I, [P, I]),
     "pf_validate": (I, [P]),
     "pf_density": (I, [P]),
     "pf_residuals": (I, [P]),
@@ -69,6 +70,14 @@ SIGNATURES = {
     "pf_domain_stress": (I, [P, I, DP, I, DP]),
     "pf_domain_destroy": (I, [P]),
     "pf_scft": (I, [P, P, P, C.POINTER(ScftOptions), C.POINTER(ScftResults)]),
+    "pf_batch_create": (I, [PP, C.POINTER(C.c_void_p), I]),
+    "pf_batch_destroy": (I, [P]),
+    "pf_batch_density": (I, [P]),
+    "pf_batch_residuals": (I, [P]),
+    "pf_batch_set_precision": (I, [P, I]),
+    "pf_batch_scft": (I, [P, I, I, D, I, D, IP, C.POINTER(ScftResults)]),
+    "pf_batch_timing": (I, [P, I]),
+    "pf_batch_timing_read": (I, [P, I, DP, IP]),
     "pf_system_timing": (I, [P, I]),
     "pf_system_timing_read": (I, [P, I, DP, IP]),
 }

@@ -428,6 +428,11 @@ class FieldSystem:
     def synchronize(self):
         call("pf_system_synchronize", self._h)
 
+    def set_precision(self, precision):
+        """Switch the propagator between "fp64" and "fp32" (state is kept)."""
+        call("pf_system_set_precision", self._h, {"fp64": 0, "fp32": 1}[precision])
+        self.precision = precision
+
     def timing(self, on):
         call("pf_system_timing", self._h, int(bool(on)))
 
@@ -590,3 +595,58 @@ def scft(sys, updater, domain=None, opts=None):
     if domain is not None:
         domain.system = sys
     return SCFTResults(r.time, r.nsteps, bool(r.converged), r.ferr, r.serr, r.fhelm)
+
+
+class Batch:
+    """Many FieldSystems advanced together on one GPU (no counterpart in the reference;
+    it runs the reference's scft! for each system).  Systems must share one precision
+    and stay alive while the batch exists."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        arr = (C.c_void_p * len(self.systems))(*[s._h.value for s in self.systems])
+        h = C.c_void_p()
+        call("pf_batch_create", C.byref(h), arr, len(self.systems))
+        self._h = h
+
+    def density_update(self):
+        call("pf_batch_density", self._h)
+
+    def timing(self, on):
+        call("pf_batch_timing", self._h, int(bool(on)))
+
+    def timing_read(self, kind):
+        ms, cnt = C.c_double(), C.c_int()
+        call("pf_batch_timing_read", self._h, int(kind), C.byref(ms), C.byref(cnt))
+        return ms.value, cnt.value
+
+    def residuals_update(self):
+        call("pf_batch_residuals", self._h)
+
+    def set_precision(self, precision):
+        """Switch every system's propagator between "fp64" and "fp32"."""
+        call("pf_batch_set_precision", self._h, {"fp64": 0, "fp32": 1}[precision])
+        for s in self.systems:
+            s.precision = precision
+
+    def scft(self, nhist=10, iter_euler=1, lam_euler=0.01, nsteps=100, ftol=1e-5, mask=None):
+        """scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler)) for every system
+        (fixed cell), or for those with a true ``mask`` entry; returns one SCFTResults per
+        system (ferr NaN: diverged, or not run)."""
+        res = (_CR * len(self.systems))()
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+            assert m.shape == (len(self.systems),)
+        call("pf_batch_scft", self._h, int(nhist), int(iter_euler), float(lam_euler),
+             int(nsteps), float(ftol), iptr(m) if m is not None else None, res)
+        return [SCFTResults(r.time, r.nsteps, bool(r.converged), r.ferr, r.serr, r.fhelm)
+                for r in res]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                call("pf_batch_destroy", self._h)
+            except Exception:
+                pass
+            self._h = None

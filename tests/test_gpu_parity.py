@@ -38,6 +38,12 @@ CASES = [
     ("cubic8", (8, 8, 8), 3, "cubic", (4.0,), dict(f=0.35)),
     ("ortho12x6x6", (12, 6, 6), 3, "orthorhombic", (5.0, 4.0, 3.5), dict(f=0.4)),
     ("tric8x6x4", (8, 6, 4), 3, "triclinic", (6.0, 5.0, 4.0, 1.3, 1.4, 1.2), dict(f=0.5)),
+    # sizes served by the compile-time specialised kernels
+    ("lam128", (128, 1, 1), 1, "lamellar", (12.0,), dict(f=0.5)),
+    ("hex48", (48, 48, 1), 2, "hexagonal", (7.0,), dict(f=0.35)),
+    ("hex96", (96, 96, 1), 2, "hexagonal", (9.0,), dict(f=0.3)),
+    ("cubic32", (32, 32, 32), 3, "cubic", (5.0,), dict(f=0.35)),
+    ("cubic64", (64, 64, 64), 3, "cubic", (8.0,), dict(f=0.35)),
 ]
 
 
@@ -45,7 +51,8 @@ CASES = [
 def test_density_stress_residuals_fp64(name, dims, dim, crystal, params, kw):
     def build(mod, precision=None):
         extra = {} if precision is None else {"precision": precision}
-        return diblock_system(mod, dims, dim, crystal, params, Ns=40, **kw, **extra)
+        return diblock_system(mod, dims, dim, crystal, params, Ns=20 if dims[2] == 64 else 40,
+                              **kw, **extra)
 
     shape = (dims[2], dims[1], dims[0])
     (so, _, co), (sg, _, cg) = both(build, rand_fields(shape))
@@ -176,7 +183,10 @@ def test_hex_cylinder_notebook_state_on_gpu():
     assert res.fhelm == pytest.approx(0.10659231121049459, abs=1e-7)
 
 
-@pytest.mark.parametrize("name,dims,dim,crystal,params,kw", CASES[:4], ids=[c[0] for c in CASES[:4]])
+FP32_CASES = [CASES[i] for i in (0, 1, 2, 3, 7, 8, 9, 10)]
+
+
+@pytest.mark.parametrize("name,dims,dim,crystal,params,kw", FP32_CASES, ids=[c[0] for c in FP32_CASES])
 def test_fp32_propagator_error_vs_fp64(name, dims, dim, crystal, params, kw):
     def build(mod, precision=None):
         extra = {} if precision is None else {"precision": precision}
@@ -261,3 +271,77 @@ def test_domain_step_from_identical_state_fp64():
         P.fieldinit(sg, fields={m: so.fields[m] for m in (1, 2)})
         dg.scfstress_update()
         np.testing.assert_allclose(dg.stress, do.stress, rtol=1e-9, err_msg=str(it))
+
+
+def _hex_system(mod, n, a, fA_N, chiN, Ns, precision=None):
+    extra = {} if precision is None else {"precision": precision}
+    cell = mod.UnitCell(2, "hexagonal", a)
+    A, B = mod.Monomer(1), mod.Monomer(2)
+    ch = mod.Multiblock([A, B], [fA_N, 100 - fA_N], [1.0, 1.0], Ns)
+    itx = mod.FloryInteraction()
+    itx.set_interaction(1, 2, chiN / 100.0)
+    sys = mod.FieldSystem((n, n, 1), cell, monomers=[A, B], **extra)
+    sys.add_species(ch, 1.0)
+    sys.add_interaction(itx)
+    return sys
+
+
+BATCH = [(32, 4.5, 30, 14.0, 64), (48, 5.5, 35, 18.0, 80), (64, 6.5, 40, 22.0, 100),
+         (96, 7.5, 28, 26.0, 128), (32, 5.0, 45, 12.0, 70)]
+
+
+def test_batch_density_mixed_grids_fp64():
+    sg = [_hex_system(P, *c, precision="fp64") for c in BATCH]
+    so = [_hex_system(O, *c) for c in BATCH]
+    for k, (g, o) in enumerate(zip(sg, so)):
+        f = rand_fields((1, BATCH[k][0], BATCH[k][0]), seed=k)
+        for m in (1, 2):
+            g.fields[m] = f[m]
+            o.fields[m] = f[m]
+    b = P.Batch(sg)
+    b.density_update()
+    for g, o in zip(sg, so):
+        o.density_update()
+        assert g.species[0].Q == pytest.approx(o.species[0].Q, rel=1e-12)
+        for m in (1, 2):
+            assert rel(g.density[m], o.density[m]) < 1e-12
+
+
+def test_batch_scft_matches_one_at_a_time_fp64():
+    def make():
+        out = []
+        for k, c in enumerate(BATCH):
+            s = _hex_system(P, *c, precision="fp64")
+            P.fieldinit(s, seed=100 + k, scale=0.1)
+            out.append(s)
+        return out
+
+    single = make()
+    ref = [P.scft(s, P.AndersonUpdater(nhist=10), opts=P.SCFTOptions(nsteps=60, ftol=1e-6))
+           for s in single]
+    batch_sys = make()
+    res = P.Batch(batch_sys).scft(nhist=10, nsteps=60, ftol=1e-6)
+    for r, q in zip(res, ref):
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-13)
+        assert r.ferr == pytest.approx(q.ferr, rel=1e-12)
+
+
+def test_batch_3d_density_fp32_matches_single():
+    def make():
+        return diblock_system(P, (32, 32, 32), 3, "cubic", (5.0,), f=0.35, Ns=40,
+                              precision="fp32")[0]
+    singles = [make() for _ in range(3)]
+    batched = [make() for _ in range(3)]
+    for k in range(3):
+        f = rand_fields((32, 32, 32), seed=k)
+        for m in (1, 2):
+            singles[k].fields[m] = f[m]
+            batched[k].fields[m] = f[m]
+        singles[k].density_update()
+    P.Batch(batched).density_update()
+    for a, b in zip(singles, batched):
+        assert b.species[0].Q == pytest.approx(a.species[0].Q, rel=1e-6)
+        for m in (1, 2):
+            assert rel(b.density[m], a.density[m]) < 1e-6
