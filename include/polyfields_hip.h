@@ -108,6 +108,11 @@ int pf_system_set_flory(pf_system* sys, int mid_a, int mid_b, double chi);
 int pf_system_set_edwards(pf_system* sys, int mid, double u);
 int pf_system_set_field(pf_system* sys, int mid, const double* src);
 int pf_system_get_grid(pf_system* sys, int which, int mid, double* dst);
+/* Seeding from densities (fieldinit! with field_type = :rho, fileio.jl:25-54): write
+   sys.density[mid], then pf_uniform_fields sets w = potentials(rho), mean-subtracted in
+   the canonical ensemble (uniform_fields!, system/system.jl:348-357). */
+int pf_system_set_density(pf_system* sys, int mid, const double* src);
+int pf_uniform_fields(pf_system* sys);
 int pf_system_get_species_density(pf_system* sys, int species, int mid, double* dst);
 int pf_system_get_propagator(pf_system* sys, int species, int s, int backward, double* dst);
 int pf_system_species_scalars(pf_system* sys, int species, double* Q, double* phi, double* mu);
@@ -118,6 +123,9 @@ int pf_system_synchronize(pf_system* sys);
    every sum are float64 in both modes, so the state carries over; the q stacks are
    reallocated.  Used to polish an FP32-converged state in FP64. */
 int pf_system_set_precision(pf_system* sys, int precision);
+/* Propagator scheme: 0 RK2 (default), 1 RQM4 (mde.jl:83-104; 1-D and 2-D grids).
+   Replaces the `mde` keyword of FieldSystem (system/system.jl:51-56). */
+int pf_system_set_mde(pf_system* sys, int method);
 
 int pf_validate(pf_system* sys);
 int pf_density(pf_system* sys);

@@ -165,6 +165,22 @@ int pf_system_set_field(pf_system* sys, int mid, const double* src) {
     });
 }
 
+int pf_system_set_density(pf_system* sys, int mid, const double* src) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(src, "density source");
+        const int m = s.slot_of(mid);
+        need(m >= 0, "monomer id");
+        PF_HIP(hipMemcpyAsync(s.rho + (size_t)m * s.n, src, sizeof(double) * s.n,
+                              hipMemcpyHostToDevice, s.stream));
+        s.sync();
+    });
+}
+
+int pf_uniform_fields(pf_system* sys) {
+    return guard([&] { S(sys).uniform_fields(); });
+}
+
 int pf_system_get_grid(pf_system* sys, int which, int mid, double* dst) {
     return guard([&] {
         auto& s = S(sys);
@@ -231,6 +247,17 @@ int pf_system_cell(pf_system* sys, double* params, int nparams, int set) {
 
 int pf_system_synchronize(pf_system* sys) {
     return guard([&] { S(sys).sync(); });
+}
+
+int pf_system_set_mde(pf_system* sys, int method) {
+    return guard([&] {
+        need(method == 0 || method == 1, "mde method (0 RK2, 1 RQM4)");
+        auto& s = S(sys);
+        if (s.mde != method) {
+            s.mde = method;
+            s.dirty_species = 1;
+        }
+    });
 }
 
 int pf_system_set_precision(pf_system* sys, int precision) {

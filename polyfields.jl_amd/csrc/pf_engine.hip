@@ -42,6 +42,7 @@ class Engine : public EngineBase {
         T* q = nullptr;
         T* qc = nullptr;
         std::vector<T*> lw;
+        std::vector<T*> lw2;       // RQM4 half-step tables
         int row0 = 0;              // first stress row of this species
     };
     System* S = nullptr;
@@ -81,6 +82,7 @@ class Engine : public EngineBase {
             dev_free(b.q);
             dev_free(b.qc);
             for (T* p : b.lw) dev_free(p);
+            for (T* p : b.lw2) dev_free(p);
         }
         sb.clear();
         for (auto* k : kbufs) dev_free(k);
@@ -129,6 +131,10 @@ class Engine : public EngineBase {
         tile = (nxh + ntiles - 1) / ntiles;
         lds_plane = (int)(sizeof(cpx<T>) * ny * nxh);
         lds_pencil = (int)(sizeof(cpx<T>) * nz * tile);
+        if (s.mde == 1 && !planar)
+            throw Error(-2, "the RQM4 propagator is provided for 1-D and 2-D grids only");
+        if (s.mde == 1 && 2 * lds_plane > 160 * 1024)
+            throw Error(-2, "RQM4 needs two xy planes in LDS; this plane is too large");
         if (lds_plane > 160 * 1024)
             throw Error(-2, "xy plane of " + std::to_string(nx) + "x" + std::to_string(ny) +
                                 " does not fit the 160 KiB LDS of one workgroup");
@@ -146,6 +152,8 @@ class Engine : public EngineBase {
             b.q = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
             if (sp.nblk() > 1) b.qc = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
             for (int k = 0; k < sp.nblk(); ++k) b.lw.push_back((T*)dev_alloc(sizeof(T) * n));
+            if (s.mde == 1)
+                for (int k = 0; k < sp.nblk(); ++k) b.lw2.push_back((T*)dev_alloc(sizeof(T) * n));
             maxNs = std::max(maxNs, sp.Ns);
             for (int d = 0; d < (sp.nblk() > 1 ? 2 : 1); ++d) {
                 Lane<T> L{};
@@ -156,8 +164,10 @@ class Engine : public EngineBase {
                 L.nsteps = sp.Ns;
                 L.nblk = sp.nblk();
                 for (int k = 0; k <= sp.nblk(); ++k) L.begin[k] = sp.begin[k];
+                L.rqm4 = s.mde == 1;
                 for (int k = 0; k < sp.nblk(); ++k) {
                     L.lw[k] = b.lw[k];
+                    if (s.mde == 1) L.lw2[k] = b.lw2[k];
                     const double bb = sp.b_block[k];
                     L.ldc[k] = sp.ds_block[k] * (bb * bb) / 6.0;
                 }
@@ -289,6 +299,7 @@ class Engine : public EngineBase {
                 const int slot = sp.block_slot[k];
                 const double coef = (-(sp.ds_block[k] / 2.0)) * s.mon_vol[slot];
                 launch_lw_table<T>(sb[i].lw[k], s.w + (size_t)slot * n, coef, n, st);
+                if (s.mde == 1) launch_lw_half_table<T>(sb[i].lw2[k], s.w + (size_t)slot * n, coef, n, st);
             }
             launch_fill<T>(sb[i].q, T(1), n, st);
             if (sb[i].qc) launch_fill<T>(sb[i].qc + (size_t)sp.Ns * n, T(1), n, st);
@@ -297,7 +308,9 @@ class Engine : public EngineBase {
         if (nl) {
             if (planar) {
                 timed(2, [&] {
-                    if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
+                    if (s.mde == 1)
+                        launch_planar_steps<T>(d_lanes, nl, 0, maxNs, 2 * lds_plane, 256, st);
+                    else if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
                         launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st);
                 });
             } else {

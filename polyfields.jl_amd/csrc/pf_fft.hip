@@ -173,6 +173,56 @@ __global__ void __launch_bounds__(256) k_planar_steps(const Lane<T>* __restrict_
             S[y * ld + (i - y * M)] = qin[i];
         }
     }
+    if (L->rqm4) {
+        // RQM4 (src/system/mde.jl:83-104): q1 = full step, q2 = two half steps with LW1 in
+        // the middle, q_out = (4 q2 - q1) / 3.  S2 is a second plane of LDS.
+        cpx<T>* S2 = S + ny * ld;
+        const int nkp = ny * (M + 1);
+        for (int t = t0; t < tend; ++t) {
+            const int b = step_block(L, t, &s_in);
+            const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b]);
+            const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw2[b]);
+            const double mc = -L->ldc[b];
+            __syncthreads();
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                const int y = i / M, o = y * ld + (i - y * M);
+                const cpx<T> w = lw[i], w2 = lw2[i], v = S[o];
+                S2[o] = {w2.x * v.x, w2.y * v.y};
+                S[o] = {w.x * v.x, w.y * v.y};
+            }
+            __syncthreads();
+            for (int pass = 0; pass < 3; ++pass) {
+                cpx<T>* X = pass == 0 ? S : S2;
+                const bool half = pass > 0;
+                plane_forward(X, M, ny, ld, L->px, L->py, L->twx, L->twy);
+                for (int i = threadIdx.x; i < nkp; i += blockDim.x) {
+                    const int y = i / (M + 1), kx = i - y * (M + 1);
+                    const double a = mc * ksq_at(L->G, (double)kx, (double)fold(y, ny), 0.0);
+                    const T f = exp_t<T>(half ? a / 2.0 : a);
+                    X[y * ld + kx].x *= f;
+                    X[y * ld + kx].y *= f;
+                }
+                __syncthreads();
+                plane_inverse(X, M, ny, ld, L->px, L->py, L->twx, L->twy);
+                const cpx<T>* wm = pass == 2 ? lw2 : lw;   // q1 *= LW1; mid *= LW1; q2 *= LW2
+                for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                    const int y = i / M, o = y * ld + (i - y * M);
+                    const cpx<T> w = wm[i];
+                    X[o] = {(X[o].x * scale) * w.x, (X[o].y * scale) * w.y};
+                }
+                __syncthreads();
+            }
+            cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n);
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                const int y = i / M, o = y * ld + (i - y * M);
+                const cpx<T> q1 = S[o], q2 = S2[o];
+                const cpx<T> v = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+                S[o] = v;
+                qout[i] = v;
+            }
+        }
+        return;
+    }
     for (int t = t0; t < tend; ++t) {
         const int b = step_block(L, t, &s_in);
         const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b]);

@@ -51,6 +51,7 @@ struct System {
     size_t n = 0;
     int dim = 0;
     int precision = 0;               // 0: FP64 everywhere, 1: FP32 propagator + FP64 sums
+    int mde = 0;                     // 0 RK2, 1 RQM4 (src/system/mde.jl:18-30)
     int canonical = 1;
     int compress_mode = 0;           // 0 strict, 1 compressible, 2 unconstrained
     double inv_zeta = 0.0;
@@ -107,6 +108,7 @@ struct System {
     void validate();
     void density();                  // density!(sys)
     void residuals();                // residuals!(sys)
+    void uniform_fields();           // uniform_fields!(sys)
     void sys_sums(double out[5]);    // [res^2, w^2, rho*w, flory, edwards]
     double field_error();
     double free_energy();

@@ -59,6 +59,8 @@ struct Lane {
     const cpx<T>* twy;       // exp(-2 pi i m / ny)
     const cpx<T>* twz;       // exp(-2 pi i m / nz)
     FftPlan px, py, pz;      // plans for nx/2, ny, nz
+    int rqm4;                // 1: RQM4 step (Richardson of one full and two half steps)
+    const T* lw2[PF_MAXBLK]; // exp(-ds_b/4 v w) per block (RQM4 only)
 };
 
 // Stress task: transforms of q_s and q+_s reduced against dksq.
@@ -177,6 +179,8 @@ struct MixCoef {
 int reduce_parts(size_t n);
 template <typename T>
 void launch_lw_table(T* out, const double* w, double coef, size_t n, hipStream_t st);
+template <typename T>
+void launch_lw_half_table(T* out, const double* w, double coef, size_t n, hipStream_t st);
 template <typename T>
 void launch_fill(T* p, T v, size_t n, hipStream_t st);
 template <typename T>

@@ -56,6 +56,21 @@ void launch_lw_table(T* out, const double* w, double coef, size_t n, hipStream_t
     PF_HIP(hipGetLastError());
 }
 
+// LW2 = exp(-lw_coeff * vol * omega / 2) (src/system/mde.jl:51)
+template <typename T>
+__global__ void k_lw_half_table(T* out, const double* __restrict__ w, double coef, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (T)exp(coef * w[i] / 2.0);
+}
+
+template <typename T>
+void launch_lw_half_table(T* out, const double* w, double coef, size_t n, hipStream_t st) {
+    int g = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_lw_half_table<T>, dim3(g), dim3(256), 0, st, out, w, coef, n);
+    PF_HIP(hipGetLastError());
+}
+
 template <typename T>
 __global__ void k_fill(T* p, T v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -481,6 +496,7 @@ void launch_segsum(const double* in, int nseg, int nparts, int stride, int nvals
 
 #define PF_INST2(T)                                                                         \
     template void launch_lw_table<T>(T*, const double*, double, size_t, hipStream_t);        \
+    template void launch_lw_half_table<T>(T*, const double*, double, size_t, hipStream_t);   \
     template void launch_fill<T>(T*, T, size_t, hipStream_t);                                \
     template void launch_species_Q<T>(const SimpsonTask<T>*, int, size_t, double*, int,     \
                                       hipStream_t);                                         \

@@ -340,6 +340,28 @@ void System::residuals() {
     launch_residuals(residual_args(), n, partial, reduce_parts(n), sums, stream);
 }
 
+void System::uniform_fields() {
+    // src/system/system.jl:348-357: w = potentials(rho); canonical: w -= mean(w).  The
+    // residual kernel computes the potentials from the current densities (its eta/res
+    // outputs are discarded and recomputed by the residuals! that follows a seeding).
+    ResidualArgs a = residual_args();
+    a.canonical = 0;
+    launch_residuals(a, n, partial, reduce_parts(n), sums, stream);
+    std::vector<double> hp(n * nmon());
+    PF_HIP(hipMemcpyAsync(hp.data(), pot, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, stream));
+    sync();
+    if (canonical)
+        for (int m = 0; m < nmon(); ++m) {
+            double s = 0.0;
+            for (size_t i = 0; i < n; ++i) s += hp[m * n + i];
+            const double mean = s / (double)n;
+            for (size_t i = 0; i < n; ++i) hp[m * n + i] -= mean;
+        }
+    PF_HIP(hipMemcpyAsync(w, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice, stream));
+    sync();
+    fields_set = true;
+}
+
 void System::sys_sums(double out[5]) {
     launch_sys_reduce(residual_args(), n, partial, reduce_parts(n), sums + 32, stream);
     PF_HIP(hipMemcpyAsync(pinned, sums + 32, sizeof(double) * 5, hipMemcpyDeviceToHost, stream));

@@ -6,3 +6,4 @@ from .api import (CANONICAL, GRAND, AndersonUpdater, Batch, Compressibility, Dom
                   Homopolymer, Diblock, MomentumUpdater, Monomer, Multiblock, Point,
                   SCFTOptions, SCFTResults, UnitCell, chain_grid, fieldinit, scft,
                   set_interaction, updater_with_system)
+from .fileio import interpolate_grid, loadfields, savefields  # noqa: F401,E402

@@ -44,12 +44,15 @@ SIGNATURES = {
     "pf_system_set_edwards": (I, [P, I, D]),
     "pf_system_set_field": (I, [P, I, DP]),
     "pf_system_get_grid": (I, [P, I, I, DP]),
+    "pf_system_set_density": (I, [P, I, DP]),
+    "pf_uniform_fields": (I, [P]),
     "pf_system_get_species_density": (I, [P, I, I, DP]),
     "pf_system_get_propagator": (I, [P, I, I, I, DP]),
     "pf_system_species_scalars": (I, [P, I, DP, DP, DP]),
     "pf_system_cell": (I, [P, DP, I, I]),
     "pf_system_synchronize": (I, [P]),
     "pf_system_set_precision": (I, [P, I]),
+    "pf_system_set_mde": (I, [P, I]),
     "pf_validate": (I, [P]),
     "pf_density": (I, [P]),
     "pf_residuals": (I, [P]),

@@ -298,8 +298,8 @@ class FieldSystem:
     def __init__(self, dims, cell, monomers=(), ensemble=CANONICAL, compress=None, mde="RK2",
                  precision="fp64", device=0):
         dims = tuple(int(d) for d in dims)
-        if mde != "RK2":
-            raise ValueError(f"Invalid propagator method `{mde}` for this backend (RK2 only).")
+        if mde not in ("RK2", "RQM4"):
+            raise ValueError(f"Invalid propagator method `{mde}` for PseudoSpectralSolver.")
         self.dims = dims
         self.shape = (dims[2], dims[1], dims[0])
         self.ensemble = ensemble
@@ -312,6 +312,9 @@ class FieldSystem:
              cell.crystal.encode(), dptr(p), len(p), 0 if ensemble == CANONICAL else 1,
              self.compress.inv_zeta, {"fp64": 0, "fp32": 1}[precision], int(device))
         self._h = h
+        if mde == "RQM4":
+            call("pf_system_set_mde", h, 1)
+        self.mde = mde
         cell._sys = self
         self.monomers = {}
         self.species = []
@@ -442,10 +445,18 @@ class FieldSystem:
         return ms.value, cnt.value
 
 
-def fieldinit(sys, seed=-1, scale=0.1, fields=None):
-    """``fieldinit!(sys; seed, scale)`` / dict form (src/fileio.jl:14-54): seeded
-    scale*N(0,1) fields (numpy ``default_rng``, one grid per monomer in ascending id), or
-    the given grids, followed by ``residuals!``."""
+def fieldinit(sys, source=None, seed=-1, scale=0.1, fields=None, field_type="omega",
+              load_cell=True, interpolate=True):
+    """``fieldinit!`` (src/fileio.jl:14-64).  Forms: ``fieldinit(sys, seed=, scale=)`` --
+    scale*N(0,1) fields from numpy's ``default_rng(seed)``, one grid per monomer in
+    ascending id; ``fieldinit(sys, {mid: grid}, field_type=, interpolate=)``;
+    ``fieldinit(sys, "file.json", load_cell=, interpolate=)``.  ``fields=`` is the dict
+    form without interpolation.  Each form ends with ``residuals!``."""
+    from . import fileio
+    if isinstance(source, str):
+        return fileio.fieldinit_file(sys, source, load_cell=load_cell, interpolate=interpolate)
+    if isinstance(source, dict):
+        return fileio.fieldinit_dict(sys, source, field_type=field_type, interpolate=interpolate)
     if fields is None:
         rng = np.random.default_rng(None if seed < 0 else seed)
         for mid in sys.mids:

@@ -35,3 +35,7 @@ if __name__ == "__main__":
     for n in ("lam1d", "cyl2d", "abc_blend_1d"):
         convert(n)
         print("wrote", n)
+    # one seed file kept verbatim as a fixture of the JSON field-file format
+    import shutil
+    shutil.copyfile(os.path.join(REF, "lam1d.json"), os.path.join(HERE, "seed_lam1d.json"))
+    print("copied lam1d.json")

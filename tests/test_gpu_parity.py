@@ -345,3 +345,66 @@ def test_batch_3d_density_fp32_matches_single():
         assert b.species[0].Q == pytest.approx(a.species[0].Q, rel=1e-6)
         for m in (1, 2):
             assert rel(b.density[m], a.density[m]) < 1e-6
+
+
+@pytest.mark.parametrize("dims,dim,crystal,params", [((32, 1, 1), 1, "lamellar", (9.0,)),
+                                                     ((24, 12, 1), 2, "rectangular", (7.0, 4.0))])
+def test_rqm4_propagator_fp64(dims, dim, crystal, params):
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, dims, dim, crystal, params, Ns=40, mde="RQM4", **extra)
+
+    (so, _, co), (sg, _, cg) = both(build, rand_fields((dims[2], dims[1], dims[0])))
+    assert cg.Q == pytest.approx(co.Q, rel=1e-12)
+    for s in (1, co.Ns // 2, co.Ns):
+        assert rel(cg.q[s], co.q[s]) < 1e-12
+        assert rel(cg.qc[s], co.qc[s]) < 1e-12
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 1e-12
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
+
+
+def test_fieldinit_from_reference_file_and_roundtrip(tmp_path):
+    import os
+    from helpers import GOLDEN
+    path = os.path.join(GOLDEN, "seed_lam1d.json")
+    # the lam1d seed is a converged 64-point lamellar state of the chi = 0.25 diblock
+    sg, cg, _ = diblock_system(P, (64, 1, 1), 1, "lamellar", (5.0,), chi=0.25)
+    P.fieldinit(sg, path)
+    assert cg.params[0] == pytest.approx(13.100000015310233)
+    so, co, _ = diblock_system(O, (64, 1, 1), 1, "lamellar", (13.100000015310233,), chi=0.25)
+    d, f = load_seed("lam1d")
+    O.fieldinit(so, fields=f)
+    assert sg.field_error() == pytest.approx(so.field_error(), rel=1e-9)
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
+    assert so.field_error() < 1e-5
+    out = str(tmp_path / "w.json")
+    P.savefields(sg, out)
+    back = P.loadfields(out)
+    for m in (1, 2):
+        np.testing.assert_array_equal(back["fields"][m], sg.fields[m])
+    # interpolated seeding onto a finer grid, as examples.ipynb does (64 -> 128)
+    sh, ch, _ = diblock_system(P, (128, 1, 1), 1, "lamellar", (5.0,), chi=0.25)
+    P.fieldinit(sh, path)
+    s2, c2, _ = diblock_system(O, (128, 1, 1), 1, "lamellar", (13.100000015310233,), chi=0.25)
+    import oracle.fileio as OF
+    O.fieldinit(s2, fields={m: OF.interpolate_grid(f[m], (128, 1, 1)) for m in (1, 2)})
+    assert sh.field_error() == pytest.approx(s2.field_error(), rel=1e-9)
+    assert sh.free_energy() == pytest.approx(s2.free_energy(), rel=1e-12)
+
+
+def test_fieldinit_from_densities_fp64():
+    # field_type = :rho -> uniform_fields!(sys) (system.jl:348-357), then residuals!
+    rng = np.random.default_rng(11)
+    rho = {1: 0.5 + 0.2 * rng.standard_normal((1, 16, 16))}
+    rho[2] = 1.0 - rho[1]
+    sg, _, _ = diblock_system(P, (16, 16, 1), 2, "square", (6.0,), chi=0.3)
+    so, _, _ = diblock_system(O, (16, 16, 1), 2, "square", (6.0,), chi=0.3)
+    P.fieldinit(sg, rho, field_type="rho")
+    for m in (1, 2):
+        so.density[m] = rho[m].copy()
+    so.uniform_fields()
+    so.residuals_update()
+    for m in (1, 2):
+        assert rel(sg.fields[m], so.fields[m]) < 1e-12
+        assert rel(sg.density[m], so.density[m]) < 1e-12

@@ -165,3 +165,23 @@ def test_homogeneous_melt_free_energy():
     sys.residuals_update()
     assert sys.free_energy() == pytest.approx(0.2 * 0.25, rel=1e-13)
     assert sys.field_error() == pytest.approx(0.0, abs=1e-6)
+
+
+def test_rqm4_fourth_order_and_symmetric():
+    # RQM4 (mde.jl:83-104): Richardson of one full and two half Strang steps is fourth
+    # order in ds (successive error ratios climb towards 16; RK2 gives 4), and remains a
+    # symmetric operator (<q_s q+_s> = Q for every s).
+    Qs = []
+    for Ns in (40, 80, 160, 320):
+        sys, cell, ch = diblock_system(O, (32, 1, 1), 1, "lamellar", (9.0,), Ns=Ns, mde="RQM4")
+        x = np.arange(32) / 32
+        sys.fields[1] = (0.03 * np.cos(2 * np.pi * x) + 0.01 * np.sin(4 * np.pi * x)).reshape(1, 1, 32)
+        sys.fields[2] = -sys.fields[1]
+        sys.density_update()
+        Qs.append(ch.Q)
+        if Ns == 40:
+            for s in range(ch.Ns + 1):
+                assert np.sum(ch.q[s] * ch.qc[s]) / sys.ngrid == pytest.approx(ch.Q, rel=1e-12)
+    e = [abs(Qs[i + 1] - Qs[i]) for i in range(3)]
+    r = [e[0] / e[1], e[1] / e[2]]
+    assert 10.0 < r[0] < r[1] < 17.0
