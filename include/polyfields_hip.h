@@ -43,6 +43,14 @@
  *   pf_cell_geometry ........ make_basis / basis! / dbasis! / ddbasis!          system/basis.jl:11-116,
  *                                                                               system/cell.jl:113-258
  *   pf_chain_grid ........... chain_grid(N_block, Ns)                           species/base.jl:27-56
+ *
+ * Reverse mode (no counterpart in the reference, which has no derivative code; these
+ * differentiate its forward path and replace what a ChainRulesCore.rrule of
+ * `free_energy(sys)` would bind):
+ *   pf_fhelm_grad ........... d free_energy / d(w, cell, chi, u, phi, N_blk)    system/system.jl:402-433
+ *                             through density!(sys)                             system/system.jl:266-289
+ *   pf_density_vjp .......... vector-Jacobian product of density!(sys)         species/multiblock.jl:125-231
+ *   pf_species_set_blocks ... continuous block lengths on a fixed contour grid  species/base.jl:27-56
  */
 #ifndef POLYFIELDS_HIP_H
 #define POLYFIELDS_HIP_H
@@ -134,6 +142,28 @@ int pf_field_error(pf_system* sys, double* ferr);
 int pf_free_energy(pf_system* sys, double* fhelm);
 int pf_free_energy_bulk(pf_system* sys, double* fhelm);
 int pf_scfstress(pf_system* sys, double* stress, int nparams);
+
+/* Free-energy functionals differentiated by pf_fhelm_grad. */
+#define PF_FHELM_REFERENCE 0 /* free_energy exactly as system.jl:402-433 (mu = phi) */
+#define PF_FHELM_TEXTBOOK 1  /* the same with mu = log(phi / Q) (system.jl:248-259) */
+
+/* Free energy and its gradient at the current fields (canonical ensemble, RK2).  Runs
+   density!(sys) and a reverse sweep of every chain; the fields are used as given (not
+   mean-subtracted).  Outputs (each may be NULL): dw [nmon][n] (slot = ascending monomer
+   id, x fastest), dcell [nparams], dchi [nmon][nmon] (Flory pairs; zero elsewhere),
+   du [nmon] (Edwards parameters), dphi [nspecies], dN [ndN] = every block of every chain
+   species in species order (derivative with the contour grid held fixed, as
+   pf_species_set_blocks changes it). */
+int pf_fhelm_grad(pf_system* sys, int functional, double* fhelm, double* dw, double* dcell,
+                  int nparams, double* dchi, double* du, double* dphi, double* dN, int ndN);
+/* Vector-Jacobian product of density!(sys): cotangents rho_bar [nmon][n] of the system
+   densities and Q_bar [nspecies] of the species partition functions -> dw [nmon][n],
+   dcell [nparams], dN [ndN] (each output may be NULL). */
+int pf_density_vjp(pf_system* sys, const double* rho_bar, const double* Q_bar, double* dw,
+                   double* dcell, int nparams, double* dN, int ndN);
+/* Set the block lengths of chain species isp to continuous values N_block[nblk]; every
+   block keeps its number of contour steps and ds_b = v_b N_b / steps_b. */
+int pf_species_set_blocks(pf_system* sys, int isp, const double* N_block, int nblk);
 
 /* Field updaters.  method: Euler 0 PEC, 1 PECE; Momentum 0 MOM, 1 RMS, 2 ADAM, 3 NAG. */
 int pf_updater_anderson(pf_updater** out, int nhist, int iter_euler, double lam_euler);

@@ -48,6 +48,28 @@ pf::System& S(pf_system* p) {
 }
 }  // namespace
 
+namespace {
+int chain_blocks(const pf::System& s) {
+    int nb = 0;
+    for (const auto& sp : s.species)
+        if (sp.kind == 0) nb += sp.nblk();
+    return nb;
+}
+
+void copy_grad(const pf::System& s, const pf::GradResult& g, double* dw, double* dcell,
+               int nparams, double* dN, int ndN) {
+    if (dw) std::memcpy(dw, g.dw.data(), sizeof(double) * g.dw.size());
+    if (dcell) {
+        need(nparams == (int)s.cell.params.size(), "nparams");
+        for (int k = 0; k < nparams; ++k) dcell[k] = g.dtheta[k];
+    }
+    if (dN) {
+        need(ndN == (int)g.dN.size(), "ndN (total blocks of the chain species)");
+        for (int k = 0; k < ndN; ++k) dN[k] = g.dN[k];
+    }
+}
+}  // namespace
+
 extern "C" {
 
 const char* pf_last_error(void) { return g_err.c_str(); }
@@ -303,6 +325,44 @@ int pf_scfstress(pf_system* sys, double* stress, int nparams) {
         need(stress && nparams == (int)s.cell.params.size(), "stress output");
         auto v = s.scfstress();
         for (int k = 0; k < nparams; ++k) stress[k] = v[k];
+    });
+}
+
+
+int pf_fhelm_grad(pf_system* sys, int functional, double* fhelm, double* dw, double* dcell,
+                  int nparams, double* dchi, double* du, double* dphi, double* dN, int ndN) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(functional == 0 || functional == 1, "functional");
+        if (dcell) need(nparams == (int)s.cell.params.size(), "nparams");
+        if (dN) need(ndN == chain_blocks(s), "ndN (total blocks of the chain species)");
+        pf::GradResult g;
+        pf::fhelm_grad(s, functional, nullptr, nullptr, g);
+        if (fhelm) *fhelm = g.fhelm;
+        copy_grad(s, g, dw, dcell, nparams, dN, ndN);
+        if (dchi) std::memcpy(dchi, g.dchi.data(), sizeof(double) * g.dchi.size());
+        if (du) std::memcpy(du, g.du.data(), sizeof(double) * g.du.size());
+        if (dphi) std::memcpy(dphi, g.dphi.data(), sizeof(double) * g.dphi.size());
+    });
+}
+
+int pf_density_vjp(pf_system* sys, const double* rho_bar, const double* Q_bar, double* dw,
+                   double* dcell, int nparams, double* dN, int ndN) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(rho_bar && Q_bar, "cotangents");
+        if (dcell) need(nparams == (int)s.cell.params.size(), "nparams");
+        if (dN) need(ndN == chain_blocks(s), "ndN (total blocks of the chain species)");
+        pf::GradResult g;
+        pf::fhelm_grad(s, -1, rho_bar, Q_bar, g);
+        copy_grad(s, g, dw, dcell, nparams, dN, ndN);
+    });
+}
+
+int pf_species_set_blocks(pf_system* sys, int isp, const double* N_block, int nblk) {
+    return guard([&] {
+        need(N_block && nblk > 0, "block lengths");
+        S(sys).set_block_lengths(isp, std::vector<double>(N_block, N_block + nblk));
     });
 }
 

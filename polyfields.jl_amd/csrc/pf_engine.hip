@@ -76,8 +76,23 @@ class Engine : public EngineBase {
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evlog;
     size_t evnext = 0;
+    // reverse sweeps: one adjoint lane per forward lane, with its own scratch
+    std::vector<int> lane_sp;        // species of each forward lane
+    std::vector<Lane<T>> alanes;
+    Lane<T>* d_alanes = nullptr;
+    std::vector<void*> abuf;         // per adjoint lane: kbuf, kb2, wbar, gpart
+
+    void free_adjoint() {
+        for (void* p : abuf) dev_free(p);
+        abuf.clear();
+        alanes.clear();
+        dev_free(d_alanes);
+        d_alanes = nullptr;
+    }
 
     void free_all() {
+        free_adjoint();
+        lane_sp.clear();
         for (auto& b : sb) {
             dev_free(b.q);
             dev_free(b.qc);
@@ -183,6 +198,7 @@ class Engine : public EngineBase {
                 L.py = py;
                 L.pz = pz;
                 lanes.push_back(L);
+                lane_sp.push_back((int)i);
             }
             SimpsonTask<T> tk{};
             tk.q = b.q;
@@ -427,6 +443,135 @@ class Engine : public EngineBase {
         PF_HIP(hipStreamSynchronize(st));
     }
 
+    void block_densities(System& s, const std::vector<double*>& dst) override {
+        if (tasks.empty()) return;
+        std::vector<SimpsonTask<T>> tk = tasks;
+        for (auto& t : tk) t.rho_blk = dst[t.isp];
+        auto* d = (SimpsonTask<T>*)dev_alloc(sizeof(SimpsonTask<T>) * tk.size());
+        upload(d, tk.data(), sizeof(SimpsonTask<T>) * tk.size());
+        launch_simpson<T>(d, (int)tk.size(), s.n, s.stream);
+        PF_HIP(hipStreamSynchronize(s.stream));
+        dev_free(d);
+    }
+
+    // Reverse sweeps of every chain species (kernels in pf_adjoint.hip).  Requires the
+    // stacks and LW tables of the density() call just made.
+    void adjoint(System& s, AdjointIO& io) override {
+        if (s.dirty_species || sb.size() != s.species.size())
+            throw Error(-2, "density must run before the reverse sweeps");
+        const int nl = (int)lanes.size();
+        const size_t n = s.n;
+        const int np = (int)s.cell.params.size();
+        const int ny = s.dims[1], nz = s.dims[2];
+        const int per = ny * ntiles;
+        constexpr int NG = PF_MAXPAR + 1;
+        hipStream_t st = s.stream;
+        io.dtheta.assign(np, 0.0);
+        io.dds_ld.assign(s.species.size(), {});
+        io.wdot.assign(s.species.size(), {});
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            io.dds_ld[i].assign(s.species[i].nblk(), 0.0);
+            io.wdot[i].assign(s.species[i].nblk(), 0.0);
+        }
+        if (!nl) return;
+        if ((int)alanes.size() != nl) {
+            free_adjoint();
+            alanes.assign(nl, Lane<T>{});
+            for (int j = 0; j < nl; ++j) {
+                const int nb = lanes[j].nblk;
+                abuf.push_back(dev_alloc(sizeof(cpx<T>) * nk));
+                abuf.push_back(dev_alloc(sizeof(cpx<T>) * nk));
+                abuf.push_back(dev_alloc(sizeof(double) * n * nb));
+                abuf.push_back(dev_alloc(sizeof(double) * (size_t)per * nb * NG));
+            }
+            d_alanes = (Lane<T>*)dev_alloc(sizeof(Lane<T>) * nl);
+        }
+        for (int j = 0; j < nl; ++j) {
+            Lane<T> A = lanes[j];
+            const int i = lane_sp[j];
+            const Species& sp = s.species[i];
+            const bool homo = sb[i].qc == nullptr;
+            A.kbuf = (cpx<T>*)abuf[4 * j];
+            A.kb2 = (cpx<T>*)abuf[4 * j + 1];
+            A.wbar = (double*)abuf[4 * j + 2];
+            A.gpart = (double*)abuf[4 * j + 3];
+            A.fwd = lanes[j].stack;
+            A.other = (A.dir > 0 && !homo) ? sb[i].qc : sb[i].q;
+            A.other_rev = homo ? 1 : 0;
+            A.rbar = io.rbar;
+            const double Q = s.Q[i], phi = s.phi[i];
+            for (int k = 0; k < sp.nblk(); ++k) {
+                const int slot = sp.block_slot[k];
+                A.blk_slot[k] = slot;
+                A.srcc[k] = phi * sp.ds_block[k] / (3.0 * Q * sp.Nref) * (homo ? 2.0 : 1.0);
+                A.wco[k] = -(sp.ds_block[k] / 2.0) * s.mon_vol[slot];
+            }
+            A.qinit = A.dir > 0 ? io.qtot[i] / (double)n : 0.0;
+            A.nparams = np;
+            A.dim = s.cell.dim;
+            std::memset(A.dGG, 0, sizeof(A.dGG));
+            for (int k = 0; k < 9 * np; ++k) A.dGG[k] = s.cell.dGG[k];
+            std::memcpy(A.G, s.cell.G, sizeof(A.G));
+            alanes[j] = A;
+            launch_fill<double>(A.wbar, 0.0, n * sp.nblk(), st);
+            launch_fill<double>(A.gpart, 0.0, (size_t)per * sp.nblk() * NG, st);
+        }
+        upload(d_alanes, alanes.data(), sizeof(Lane<T>) * nl);
+        launch_adj_slab<T>(d_alanes, nl, nz, 0, 0, lds_plane, st);
+        for (int t = 0; t < maxNs; ++t) {
+            launch_adj_pencil<T>(d_alanes, nl, ny, ntiles, tile, t, 2 * lds_pencil, st);
+            launch_adj_slab<T>(d_alanes, nl, nz, 1, t, lds_plane, st);
+        }
+        // pool the per-lane, per-block field derivatives; <w, wbar_b> for the ds terms
+        DotArgs da{};
+        std::vector<std::pair<int, int>> who;
+        std::vector<double> dots;
+        const int nparts = std::min(reduce_parts(n), 256);
+        auto flush = [&] {
+            if (!da.k) return;
+            launch_dots(da, n, s.g_part, nparts, s.g_out, st);
+            std::vector<double> h(da.k);
+            PF_HIP(hipMemcpyAsync(h.data(), s.g_out, sizeof(double) * da.k,
+                                  hipMemcpyDeviceToHost, st));
+            PF_HIP(hipStreamSynchronize(st));
+            dots.insert(dots.end(), h.begin(), h.end());
+            da.k = 0;
+        };
+        for (int j = 0; j < nl; ++j) {
+            const Species& sp = s.species[lane_sp[j]];
+            for (int k = 0; k < sp.nblk(); ++k) {
+                const int slot = sp.block_slot[k];
+                const double* wb = alanes[j].wbar + (size_t)k * n;
+                launch_axpy(io.dw + (size_t)slot * n, wb, 1.0, n, st);
+                da.x[da.k] = s.w + (size_t)slot * n;
+                da.y[da.k] = wb;
+                ++da.k;
+                who.push_back({lane_sp[j], k});
+                if (da.k == 48) flush();
+            }
+        }
+        flush();
+        for (size_t q = 0; q < who.size(); ++q) io.wdot[who[q].first][who[q].second] += dots[q];
+        // k-space sums: fixed-order host reduction over the pencil workgroups
+        for (int j = 0; j < nl; ++j) {
+            const int i = lane_sp[j];
+            const Species& sp = s.species[i];
+            const int nb = sp.nblk();
+            std::vector<double> h((size_t)per * nb * NG);
+            PF_HIP(hipMemcpyAsync(h.data(), alanes[j].gpart, sizeof(double) * h.size(),
+                                  hipMemcpyDeviceToHost, st));
+            PF_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < nb; ++k) {
+                double G[NG] = {0};
+                for (int r = 0; r < per; ++r)
+                    for (int g = 0; g < NG; ++g) G[g] += h[((size_t)r * nb + k) * NG + g];
+                const double ldc = lanes[j].ldc[k];
+                for (int p = 0; p < np; ++p) io.dtheta[p] += -(ldc / (double)n) * G[p];
+                io.dds_ld[i][k] += -(ldc / (sp.ds_block[k] * (double)n)) * G[PF_MAXPAR];
+            }
+        }
+    }
+
     void get_propagator(System& s, int isp, int slice, int backward, double* host) override {
         if (sb.size() != s.species.size() || s.species[isp].kind != 0)
             throw Error(-2, "no propagator for this species (run density first)");
@@ -476,6 +621,19 @@ class BatchEngine : public BatchEngineBase {
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evlog;
     size_t evnext = 0;
+    // reverse sweeps: one adjoint lane per forward lane, with its own scratch
+    std::vector<int> lane_sp;        // species of each forward lane
+    std::vector<Lane<T>> alanes;
+    Lane<T>* d_alanes = nullptr;
+    std::vector<void*> abuf;         // per adjoint lane: kbuf, kb2, wbar, gpart
+
+    void free_adjoint() {
+        for (void* p : abuf) dev_free(p);
+        abuf.clear();
+        alanes.clear();
+        dev_free(d_alanes);
+        d_alanes = nullptr;
+    }
 
     hipEvent_t ev() {
         if (evnext >= evpool.size()) {

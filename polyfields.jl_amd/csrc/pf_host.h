@@ -117,10 +117,48 @@ struct System {
     void monomer_fractions(std::vector<double>& fr);
     void sync();
     void set_precision(int p);       // swap the propagator engine (fields stay FP64)
+    void set_block_lengths(int isp, const std::vector<double>& N_block);
+
+    // free-energy gradient scratch (allocated on first use)
+    double* g_rbar = nullptr;        // [slot][n]
+    double* g_dw = nullptr;          // [slot][n]
+    double* g_blk = nullptr;         // per-block densities of all chain species
+    size_t g_blk_len = 0;
+    double* g_part = nullptr;        // dot partials
+    double* g_out = nullptr;         // dot results
+};
+
+// Reverse-mode derivative of the free energy (pf_grad.cpp).  functional: 0 the reference's
+// free_energy (mu = phi), 1 the same with mu = log(phi / Q); -1: density VJP only, with
+// the cotangents rho_bar (host [slot][n]) and Q_bar (host, per species).
+struct GradResult {
+    double fhelm = 0.0;
+    std::vector<double> dw;          // host [slot][n]
+    std::vector<double> dtheta;      // [nparams]
+    std::vector<double> dchi;        // [nmon * nmon], Flory pairs only
+    std::vector<double> du;          // [nmon], Edwards parameters only
+    std::vector<double> dphi;        // [nspecies]
+    std::vector<double> dN;          // chain blocks in species order
+};
+void fhelm_grad(System& s, int functional, const double* rho_bar, const double* Q_bar,
+                GradResult& g);
+
+// Cotangents in and sums out of the reverse sweeps (Engine::adjoint).
+struct AdjointIO {
+    const double* rbar = nullptr;          // device [slot][n]: dF/d(system density)
+    std::vector<double> qtot;              // per species: total dF/dQ (chains)
+    double* dw = nullptr;                  // device [slot][n]: dF/dw, accumulated into
+    std::vector<double> dtheta;            // out: dF/dtheta
+    std::vector<std::vector<double>> dds_ld;  // out: LD part of dF/dds_b, per species/block
+    std::vector<std::vector<double>> wdot;    // out: <w_mon(b), wbar_b>, per species/block
 };
 
 class EngineBase {
    public:
+    // reverse sweeps of every chain species after density(); see pf_adjoint.hip
+    virtual void adjoint(System& s, AdjointIO& io) = 0;
+    // per-block Simpson densities of chain species into dst[isp] ([blk][n]; null: skip)
+    virtual void block_densities(System& s, const std::vector<double*>& dst) = 0;
     virtual ~EngineBase() = default;
     virtual void density(System& s) = 0;
     virtual void stress_rows(System& s, std::vector<double>& rows) = 0;   // [row][PF_MAXPAR]

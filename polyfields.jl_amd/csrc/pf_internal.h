@@ -61,6 +61,20 @@ struct Lane {
     FftPlan px, py, pz;      // plans for nx/2, ny, nz
     int rqm4;                // 1: RQM4 step (Richardson of one full and two half steps)
     const T* lw2[PF_MAXBLK]; // exp(-ds_b/4 v w) per block (RQM4 only)
+    // --- adjoint sweeps (free-energy gradient); unused by the density sweeps --------
+    const T* fwd;            // forward stack of the chain this lane is the adjoint of
+    const T* other;          // partner stack of the Simpson source (q+ for q, q for q+)
+    int other_rev;           // 1: partner slice s is other[Ns+2-s] (homopolymer)
+    const double* rbar;      // dF/d(system density) [slot][n]
+    int blk_slot[PF_MAXBLK]; // system monomer slot of each block
+    double srcc[PF_MAXBLK];  // phi ds_b / (3 Q Nref)  (doubled for a homopolymer)
+    double wco[PF_MAXBLK];   // -(ds_b v_b / 2)
+    double qinit;            // adjoint value at the starting slice from dF/dQ (q: Qbar/n)
+    double* wbar;            // dF/dw accumulated by this lane, per block [blk][n]
+    cpx<T>* kb2;             // nk scratch: spectrum of LW a_in
+    double* gpart;           // per-workgroup k-space sums [wg][blk][PF_MAXPAR + 1]
+    int nparams, dim;
+    double dGG[3 * 3 * PF_MAXPAR];
 };
 
 // Stress task: transforms of q_s and q+_s reduced against dksq.
@@ -95,6 +109,7 @@ struct SimpsonTask {
     int isp;                 // species index (scalar slots)
     double* rho_sp;          // species density per monomer slot: rho_sp[slot*n + i]
     int nmon;
+    double* rho_blk;         // optional: per-block densities [blk][n] (gradient pass)
     size_t n;                // grid points of the owning system
     DevScalars* sc;          // scalar slots of the owning system
 };
@@ -131,6 +146,14 @@ void launch_stress_pencil(const StressLane<T>* d_lanes, int nlanes, int ny, int 
                           hipStream_t st);
 
 enum SlabMode { SLAB_FWD = 0, SLAB_INVFWD = 1 };
+
+// adjoint sweeps (pf_adjoint.hip); mode 0 starts the sweep, 1 continues after step t
+template <typename T>
+void launch_adj_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, int lds_bytes,
+                     hipStream_t st);
+template <typename T>
+void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
+                       int lds_bytes, hipStream_t st);
 
 // compile-time specialised variants (pf_fft_fixed.hip); false: size not specialised
 template <typename T>
@@ -210,5 +233,17 @@ void launch_momentum(int method, double* mt, double* vt, double* step, const dou
                      double b1, double b2, size_t N, hipStream_t st);
 void launch_segsum(const double* in, int nseg, int nparts, int stride, int nvals, double* out,
                    hipStream_t st);
+
+// gradient helpers (pf_adjoint.hip)
+struct DotArgs {
+    int k;
+    const double* x[48];
+    const double* y[48];
+};
+void launch_dots(const DotArgs& a, size_t n, double* partial, int nparts, double* out,
+                 hipStream_t st);
+void launch_rbar(const ResidualArgs& a, size_t n, double* rbar, hipStream_t st);
+void launch_point_grad(const double* w, double vol, size_t n, const double* rbar, double ca,
+                       double cb, double* dw, hipStream_t st);
 
 }  // namespace pf

@@ -157,6 +157,7 @@ __global__ void k_simpson(const SimpsonTask<T>* __restrict__ tasks) {
             rho *= tk->ds[b] / 3.0;
             rho /= QN;
             acc_m[tk->blk_mon[b]] += rho;
+            if (tk->rho_blk) tk->rho_blk[(size_t)b * n + i] = rho;
         }
         for (int m = 0; m < tk->nmon; ++m) tk->rho_sp[(size_t)m * n + i] = acc_m[m];
     }

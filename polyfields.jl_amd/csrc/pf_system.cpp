@@ -86,6 +86,11 @@ System::~System() {
     dev_free(sc);
     dev_free(partial);
     dev_free(sums);
+    dev_free(g_rbar);
+    dev_free(g_dw);
+    dev_free(g_blk);
+    dev_free(g_part);
+    dev_free(g_out);
     if (pinned) (void)hipHostFree(pinned);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -105,6 +110,28 @@ void System::set_precision(int p) {
     eng.reset();
     precision = p;
     eng = make_engine(*this);
+    dirty_species = 1;
+}
+
+void System::set_block_lengths(int isp, const std::vector<double>& N_block) {
+    // Continuous block lengths on a fixed contour grid: every block keeps its step count
+    // (the begin indices of src/species/base.jl:27-56) and ds_b = v_b N_b / steps_b.  This
+    // is the argument the block-length derivative of the free energy is taken against.
+    if (isp < 0 || isp >= (int)species.size()) throw Error(-2, "species index out of range");
+    Species& sp = species[isp];
+    if (sp.kind != 0) throw Error(-2, "block lengths are defined for chain species only");
+    if ((int)N_block.size() != sp.nblk()) throw Error(-2, "wrong number of block lengths");
+    double Nref = 0.0, N = 0.0;
+    for (int b = 0; b < sp.nblk(); ++b) {
+        if (!(N_block[b] > 0.0)) throw Error(-2, "block lengths must be positive");
+        sp.Nref_block[b] = sp.vol_block[b] * N_block[b];
+        sp.ds_block[b] = sp.Nref_block[b] / (double)(sp.begin[b + 1] - sp.begin[b]);
+        Nref += sp.Nref_block[b];
+        N += N_block[b];
+    }
+    sp.Nref = Nref;
+    sp.N = N;
+    for (int b = 0; b < sp.nblk(); ++b) sp.f_block[b] = sp.Nref_block[b] / Nref;
     dirty_species = 1;
 }
 

@@ -7,3 +7,5 @@ from .api import (CANONICAL, GRAND, AndersonUpdater, Batch, Compressibility, Dom
                   SCFTOptions, SCFTResults, UnitCell, chain_grid, fieldinit, scft,
                   set_interaction, updater_with_system)
 from .fileio import interpolate_grid, loadfields, savefields  # noqa: F401,E402
+from .adjoint import (FUNCTIONALS, FhelmFunction, FhelmGrad, density_vjp,  # noqa: F401,E402
+                      fhelm_grad, set_block_lengths)
