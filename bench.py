@@ -179,16 +179,31 @@ def batched_scft(P, count, plain_sample):
     }
 
 
+def single_instance(P, args, nsteps_mde):
+    s1, _ = build(P, args.precision, args.seed)
+    for m, w in seeded_fields(s1.shape, args.seed).items():
+        s1.fields[m] = w
+    for _ in range(2):
+        s1.density_update()
+    s1.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s1.density_update()
+    s1.synchronize()
+    return nsteps_mde * args.steps / (time.perf_counter() - t0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--instances", type=int, default=16)
+    ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--sweep-instances", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=8)
     args = ap.parse_args()
@@ -217,6 +232,7 @@ def main():
     dt = time.perf_counter() - t0
     slab_ms, slab_n = batch.timing_read(0)
     pen_ms, pen_n = batch.timing_read(1)
+    slab_lanes, pen_lanes = batch.timing_lanes(0), batch.timing_lanes(1)
     batch.timing(False)
 
     ms_per_step = dt * 1e3 / args.steps
@@ -225,11 +241,12 @@ def main():
     n = GRID ** 3
     nk = (GRID // 2 + 1) * GRID * GRID
     lanes = 2 * args.instances
-    # algorithmic bytes per launch, all lanes: the slab pass reads the half spectrum
-    # (2nk reals) and LW (n), writes q_s (n) and the next half spectrum (2nk); the pencil
-    # pass reads and writes the half spectrum (4nk).
-    slab_bytes = s * (2 * n + 4 * nk) * lanes
-    pen_bytes = s * (4 * nk) * lanes
+    # algorithmic bytes per launch: per lane, the slab pass reads the half spectrum (2nk
+    # reals) and LW (n), writes q_s (n) and the next half spectrum (2nk); the pencil pass
+    # reads and writes the half spectrum (4nk).  A launch covers slab_lanes / pen_lanes
+    # lanes (the batch runs large 3-D groups as two halves on two streams).
+    slab_bytes = s * (2 * n + 4 * nk) * slab_lanes
+    pen_bytes = s * (4 * nk) * pen_lanes
     slab_avg = slab_ms / max(slab_n, 1) * 1e-3
     pen_avg = pen_ms / max(pen_n, 1) * 1e-3
     dom = ("xy-slab pass", slab_bytes, slab_avg) if slab_ms >= pen_ms else \
@@ -252,6 +269,7 @@ def main():
         "config": {"workload": "64^3 cubic AB diblock N=100 fA=0.35 chiN=20 Ns=102; one "
                                "density! (204 MDE steps) of every instance per step",
                    "instances": args.instances, "lanes_in_flight": lanes,
+                   "lanes_per_launch": {"slab": slab_lanes, "pencil": pen_lanes},
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
@@ -261,7 +279,11 @@ def main():
         "kernel_avg_us": {"slab": slab_avg * 1e6, "pencil": pen_avg * 1e6},
     }
     del batch, systems
+    # the same workload with one instance in flight (two lanes per launch)
+    if not args.no_single:
+        line["single_instance_steps_per_s"] = single_instance(P, args, nsteps_mde)
     if not args.no_sweep:
+
         line["batched_scft"] = batched_scft(P, args.sweep_instances, args.plain_sample)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.seed)

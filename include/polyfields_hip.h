@@ -212,6 +212,9 @@ int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int 
                   pf_scft_results* results);
 int pf_batch_timing(pf_batch* b, int on);
 int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches);
+/* Mean number of propagator lanes covered by the sampled launches of one kind (3-D
+   groups of 4 or more lanes are launched as two halves on two streams). */
+int pf_batch_timing_lanes(pf_batch* b, int kind, double* lanes);
 
 /* Measurement: per-launch HIP-event timing of the propagator kernels on the system's
    stream.  kind: 0 xy-slab pass, 1 z-pencil pass, 2 whole-plane (1-D/2-D) sweep. */

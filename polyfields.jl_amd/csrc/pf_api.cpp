@@ -553,6 +553,13 @@ int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches)
     });
 }
 
+int pf_batch_timing_lanes(pf_batch* b, int kind, double* lanes) {
+    return guard([&] {
+        need(lanes, "output");
+        *lanes = B(b).eng->timing_lanes(kind);
+    });
+}
+
 int pf_system_timing(pf_system* sys, int on) {
     return guard([&] { S(sys).eng->timing(on); });
 }

@@ -63,6 +63,7 @@ class BatchEngineBase {
     // HIP-event timing of the propagator launches; kind: 0 slab, 1 pencil, 2 planar
     virtual void timing(int on) = 0;
     virtual double timing_ms(int kind, int* count) = 0;
+    virtual double timing_lanes(int kind) = 0;   // mean lanes per sampled launch
 };
 
 struct BatchResult {

@@ -616,24 +616,15 @@ class BatchEngine : public BatchEngineBase {
     std::vector<int> cached;
     std::vector<long> cell_versions;
     hipEvent_t fork = nullptr;
+    std::vector<std::pair<hipStream_t, hipEvent_t>> streams;   // one per (sub-)group
+    // 3-D groups of at least 4 lanes run as 2 sub-groups on 2 streams: measured on MI355X
+    // (64^3 FP32, 32 systems) 2 streams give +20% over 1, 3 streams no further gain
+    static constexpr int split = 2;
     size_t nmax = 0;
     bool tim = false;
     std::vector<hipEvent_t> evpool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evlog;
     size_t evnext = 0;
-    // reverse sweeps: one adjoint lane per forward lane, with its own scratch
-    std::vector<int> lane_sp;        // species of each forward lane
-    std::vector<Lane<T>> alanes;
-    Lane<T>* d_alanes = nullptr;
-    std::vector<void*> abuf;         // per adjoint lane: kbuf, kb2, wbar, gpart
-
-    void free_adjoint() {
-        for (void* p : abuf) dev_free(p);
-        abuf.clear();
-        alanes.clear();
-        dev_free(d_alanes);
-        d_alanes = nullptr;
-    }
 
     hipEvent_t ev() {
         if (evnext >= evpool.size()) {
@@ -646,8 +637,9 @@ class BatchEngine : public BatchEngineBase {
     size_t ncalls[3] = {0, 0, 0};
     // events bracket one launch in 8 of each kind (recording every launch would add a
     // gap between dependent kernels); the sample still spans the whole timed region
+    std::vector<int> evlanes;        // lanes covered by each sampled launch
     template <typename F>
-    void timed(int cls, hipStream_t s, F&& f) {
+    void timed(int cls, hipStream_t s, int nl, F&& f) {
         if (!tim || (ncalls[cls]++ % 8) != 0) {
             f();
             return;
@@ -657,6 +649,7 @@ class BatchEngine : public BatchEngineBase {
         f();
         PF_HIP(hipEventRecord(b, s));
         evlog.push_back({cls, {a, b}});
+        evlanes.push_back(nl);
     }
 
     Engine<T>& E(int i) { return *static_cast<Engine<T>*>(B->sys[i]->eng.get()); }
@@ -717,26 +710,40 @@ class BatchEngine : public BatchEngineBase {
                 if (e.qc_stack((int)k)) fillt.push_back({e.qc_stack((int)k) + (size_t)sp.Ns * s.n, s.n});
             }
         }
+        // 3-D groups are split into `split` sub-groups on streams of their own, so that
+        // the slab launches of one sub-group overlap the pencil launches of another
         lanes.clear();
+        std::vector<Group> fin;
         for (size_t k = 0; k < g.size(); ++k) {
-            g[k].off = (int)lanes.size();
-            g[k].count = (int)by_size[k].size();
-            lanes.insert(lanes.end(), by_size[k].begin(), by_size[k].end());
-            if (k < groups.size() && groups[k].st) {
-                g[k].st = groups[k].st;
-                g[k].done = groups[k].done;
-                groups[k].st = nullptr;
-            } else {
-                PF_HIP(hipStreamCreateWithFlags(&g[k].st, hipStreamNonBlocking));
-                PF_HIP(hipEventCreateWithFlags(&g[k].done, hipEventDisableTiming));
+            const int cnt = (int)by_size[k].size();
+            const int parts = (g[k].nz > 1) ? std::max(1, std::min(split, cnt / 2)) : 1;
+            int done = 0;
+            for (int p = 0; p < parts; ++p) {
+                const int c = (cnt - done) / (parts - p);
+                Group sg = g[k];
+                sg.off = (int)lanes.size();
+                sg.count = c;
+                sg.maxNs = 0;
+                for (int j = done; j < done + c; ++j) {
+                    lanes.push_back(by_size[k][j]);
+                    sg.maxNs = std::max(sg.maxNs, by_size[k][j].nsteps);
+                }
+                done += c;
+                fin.push_back(sg);
             }
         }
-        for (auto& old : groups)
-            if (old.st) {
-                (void)hipStreamDestroy(old.st);
-                (void)hipEventDestroy(old.done);
-            }
-        groups = g;
+        while (streams.size() < fin.size()) {
+            hipStream_t st;
+            hipEvent_t e;
+            PF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            PF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            streams.push_back({st, e});
+        }
+        for (size_t k = 0; k < fin.size(); ++k) {
+            fin[k].st = streams[k].first;
+            fin[k].done = streams[k].second;
+        }
+        groups = fin;
         d_lanes = upload_vec(d_lanes, lanes);
         d_tasks = upload_vec(d_tasks, tasks);
         d_lwt = upload_vec(d_lwt, lwt);
@@ -753,18 +760,19 @@ class BatchEngine : public BatchEngineBase {
     }
 
    public:
-    explicit BatchEngine(Batch& b) : B(&b) { PF_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming)); }
+    explicit BatchEngine(Batch& b) : B(&b) {
+        PF_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    }
     ~BatchEngine() override {
         dev_free(d_lanes);
         dev_free(d_tasks);
         dev_free(d_lwt);
         dev_free(d_fillt);
         dev_free(partial);
-        for (auto& gr : groups)
-            if (gr.st) {
-                (void)hipStreamDestroy(gr.st);
-                (void)hipEventDestroy(gr.done);
-            }
+        for (auto& se : streams) {
+            (void)hipStreamDestroy(se.first);
+            (void)hipEventDestroy(se.second);
+        }
         (void)hipEventDestroy(fork);
         for (auto e : evpool) (void)hipEventDestroy(e);
     }
@@ -772,8 +780,20 @@ class BatchEngine : public BatchEngineBase {
     void timing(int on) override {
         tim = on != 0;
         evlog.clear();
+        evlanes.clear();
         evnext = 0;
         ncalls[0] = ncalls[1] = ncalls[2] = 0;
+    }
+
+    double timing_lanes(int which) override {
+        double tot = 0.0;
+        int c = 0;
+        for (size_t k = 0; k < evlog.size(); ++k)
+            if (evlog[k].first == which) {
+                tot += evlanes[k];
+                ++c;
+            }
+        return c ? tot / c : 0.0;
     }
 
     double timing_ms(int which, int* count) override {
@@ -803,21 +823,21 @@ class BatchEngine : public BatchEngineBase {
             PF_HIP(hipStreamWaitEvent(gr.st, fork, 0));
             const Lane<T>* dl = d_lanes + gr.off;
             if (gr.nz == 1) {
-                timed(2, gr.st, [&] {
+                timed(2, gr.st, gr.count, [&] {
                     if (!fixed_planar<T>(dl, gr.count, gr.nx, gr.ny, 0, gr.maxNs, gr.st))
                         launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.lds_plane, 256, gr.st);
                 });
             } else {
-                timed(0, gr.st, [&] {
+                timed(0, gr.st, gr.count, [&] {
                     if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_FWD, 0, gr.st))
                         launch_slab<T>(dl, gr.count, gr.nz, SLAB_FWD, 0, gr.lds_plane, 256, gr.st);
                 });
                 for (int t = 0; t < gr.maxNs; ++t) {
-                    timed(1, gr.st, [&] {
+                    timed(1, gr.st, gr.count, [&] {
                         if (!fixed_pencil<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, gr.ntiles, gr.tile, t, gr.st))
                             launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st);
                     });
-                    timed(0, gr.st, [&] {
+                    timed(0, gr.st, gr.count, [&] {
                         if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_INVFWD, t, gr.st))
                             launch_slab<T>(dl, gr.count, gr.nz, SLAB_INVFWD, t, gr.lds_plane, 256, gr.st);
                     });

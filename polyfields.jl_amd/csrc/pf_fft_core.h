@@ -25,15 +25,18 @@ __device__ void fft_pass(cpx<T>* buf, int N, int Ns, int nlines, int es, int ls,
     const int nthr = blockDim.x;
     const int lpr = max(1, (nthr * MAXB) / nb);
     const int twmul = (N / (Ns * R)) * twstride;
+    // strided lines (es > 1, ls == 1): consecutive threads on consecutive lines
+    const bool lf = (es > 1 && ls == 1);
     for (int l0 = 0; l0 < nlines; l0 += lpr) {
-        const int total = min(lpr, nlines - l0) * nb;
+        const int nlr = min(lpr, nlines - l0);
+        const int total = nlr * nb;
         cpx<T> v[MAXB][R];
 #pragma unroll
         for (int i = 0; i < MAXB; ++i) {
             const int b = threadIdx.x + i * nthr;
             if (b < total) {
-                const int line = l0 + b / nb;
-                const int j = b - (b / nb) * nb;
+                const int line = lf ? l0 + b % nlr : l0 + b / nb;
+                const int j = lf ? b / nlr : b - (b / nb) * nb;
                 const cpx<T>* src = buf + line * ls;
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[i][r] = src[(j + r * nb) * es];
@@ -53,8 +56,8 @@ __device__ void fft_pass(cpx<T>* buf, int N, int Ns, int nlines, int es, int ls,
         for (int i = 0; i < MAXB; ++i) {
             const int b = threadIdx.x + i * nthr;
             if (b < total) {
-                const int line = l0 + b / nb;
-                const int j = b - (b / nb) * nb;
+                const int line = lf ? l0 + b % nlr : l0 + b / nb;
+                const int j = lf ? b / nlr : b - (b / nb) * nb;
                 const int k = j % Ns;
                 dft<T, R, INV>(v[i]);
                 const int d = (j / Ns) * Ns * R + k;

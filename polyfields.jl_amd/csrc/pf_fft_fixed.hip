@@ -38,15 +38,20 @@ __device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __
     constexpr int LPR0 = (BPR * NT) / NB;
     constexpr int LPR = LPR0 < 1 ? 1 : (LPR0 > NL ? NL : LPR0);
     constexpr int TWM = (N / (NS * R)) * TWS;
+    // Strided lines (ES > 1, LS == 1): consecutive threads take consecutive lines, so a
+    // wavefront touches consecutive LDS words; otherwise consecutive butterflies.
+    constexpr bool LINE_FAST = (ES > 1 && LS == 1);
 #pragma unroll
     for (int l0 = 0; l0 < NL; l0 += LPR) {
-        const int total = ((NL - l0) < LPR ? (NL - l0) : LPR) * NB;
+        const int nlr = (NL - l0) < LPR ? (NL - l0) : LPR;
+        const int total = nlr * NB;
         cpx<T> v[BPR][R];
 #pragma unroll
         for (int i = 0; i < BPR; ++i) {
             const int b = threadIdx.x + i * NT;
             if (i * NT < total && b < total) {
-                const int line = l0 + b / NB, j = b % NB;
+                const int line = LINE_FAST ? l0 + b % nlr : l0 + b / NB;
+                const int j = LINE_FAST ? b / nlr : b % NB;
                 const cpx<T>* src = buf + line * LS + j * ES;
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[i][r] = src[r * NB * ES];
@@ -66,7 +71,8 @@ __device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __
         for (int i = 0; i < BPR; ++i) {
             const int b = threadIdx.x + i * NT;
             if (i * NT < total && b < total) {
-                const int line = l0 + b / NB, j = b % NB;
+                const int line = LINE_FAST ? l0 + b % nlr : l0 + b / NB;
+                const int j = LINE_FAST ? b / nlr : b % NB;
                 dft<T, R, INV>(v[i]);
                 const int d = (j / NS) * NS * R + (j % NS);
                 cpx<T>* dst = buf + line * LS + d * ES;
@@ -239,24 +245,38 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
     int s_in;
     int b = step_block(L, t, &s_in);
     if constexpr (MODE == SLAB_INVFWD) {
+        // the LW rows of this plane are loaded first, so that their latency overlaps the
+        // inverse transforms
+        constexpr int NP = (NX / 2) * NY, PER = (NP + NT - 1) / NT;
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE);
+        cpx<T> wr[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = k * NT + threadIdx.x;
+            if (i < NP) wr[k] = lw[i];
+        }
         lds_copy_in<T, KPLANE, NT>(S, kb);
         __syncthreads();
         cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
         const T scale = T(2.0 / (double)n);
-        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE);
         cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
                                                  (size_t)z * PLANE);
         const bool more = t + 1 < L->nsteps;
         int s2;
         const int b2 = more ? step_block(L, t + 1, &s2) : b;
         const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * PLANE);
-        for_pairs<T, NX, NY, NT>([&](int i, int o) {
-            const cpx<T> w = lw[i];
-            const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
-            qout[i] = v;
-            const cpx<T> w2 = lw2[i];
-            S[o] = {v.x * w2.x, v.y * w2.y};
-        });
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int i = k * NT + threadIdx.x;
+            if (i < NP) {
+                const int o = (i / (NX / 2)) * (NX / 2 + 1) + (i % (NX / 2));
+                const cpx<T> w = wr[k];
+                const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+                qout[i] = v;
+                const cpx<T> w2 = (b2 == b) ? w : lw2[i];
+                S[o] = {v.x * w2.x, v.y * w2.y};
+            }
+        }
         if (!more) return;
     } else {
         const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
@@ -322,6 +342,104 @@ __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ la
 }
 
 // ---------------------------------------------------------------------------------
+// Register-resident transforms: one thread holds a whole line of N <= 64 values and
+// runs N = R1 * R2 as R2 transforms of length R1, twiddles w_N^(n2 k1), and R1 transforms
+// of length R2 (n = R2 n1 + n2, k = k1 + R1 k2); every index is a compile-time constant,
+// so the line stays in VGPRs.  cos(2 pi j / 64) for j = 0..16; the other octants follow
+// by symmetry.
+constexpr double kCos64[17] = {
+    1.0, 0.9951847266721969, 0.9807852804032304, 0.9569403357322088,
+    0.9238795325112867, 0.881921264348355, 0.8314696123025452, 0.773010453362737,
+    0.7071067811865476, 0.6343932841636455, 0.5555702330196022, 0.47139673682599764,
+    0.3826834323650898, 0.2902846772544624, 0.19509032201612828, 0.0980171403295606,
+    0.0};
+
+constexpr double cos64(int j) {
+    j &= 63;
+    if (j <= 16) return kCos64[j];
+    if (j <= 32) return -kCos64[32 - j];
+    if (j <= 48) return -kCos64[j - 32];
+    return kCos64[64 - j];
+}
+constexpr double sin64(int j) { return cos64(j - 16 + 64); }
+
+// w_N^m = exp(-2 pi i m / N) (forward) or its conjugate (INV)
+template <typename T, int N, bool INV>
+__device__ __forceinline__ cpx<T> twr(int m) {
+    const int j = m * (64 / N);
+    return {(T)cos64(j), INV ? (T)sin64(j) : (T)(-sin64(j))};
+}
+
+template <typename T, int N, int R1, int R2, bool INV>
+__device__ __forceinline__ void fft_reg(cpx<T> (&v)[N]) {
+    static_assert(R1 * R2 == N && 64 % N == 0, "register transform size");
+    cpx<T> y[N];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+        cpx<T> a[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[R2 * n1 + n2];
+        dft<T, R1, INV>(a);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1)
+            y[n2 * R1 + k1] = (n2 * k1 == 0) ? a[k1] : cmul(a[k1], twr<T, N, INV>(n2 * k1));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        cpx<T> a[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) a[n2] = y[n2 * R1 + k1];
+        dft<T, R2, INV>(a);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = a[k2];
+    }
+}
+
+template <int N> struct RegSplit;
+template <> struct RegSplit<64> { static constexpr int R1 = 8, R2 = 8; };
+template <> struct RegSplit<32> { static constexpr int R1 = 8, R2 = 4; };
+template <> struct RegSplit<16> { static constexpr int R1 = 4, R2 = 4; };
+
+// z-pencil pass with one thread per (ky, kx) column: the column is read straight from the
+// half spectrum (consecutive threads read consecutive kx, ky -> one coalesced run per kz),
+// transformed, multiplied by LD, transformed back and written, with no LDS and no barrier.
+template <typename T, int NZ, int NT>
+__global__ void __launch_bounds__(NT) k_pencil_reg(const Lane<T>* __restrict__ lanes, int ny,
+                                                   int per, int t) {
+    constexpr int R1 = RegSplit<NZ>::R1, R2 = RegSplit<NZ>::R2;
+    const int lane = blockIdx.x / per;
+    const int c = (blockIdx.x - lane * per) * NT + threadIdx.x;
+    const Lane<T>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    const int nxh = (L->nx >> 1) + 1, ncol = nxh * ny;
+    if (c >= ncol) return;
+    int s_in;
+    const int b = step_block(L, t, &s_in);
+    cpx<T>* kb = L->kbuf + c;
+    cpx<T> v[NZ];
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) v[z] = kb[(size_t)z * ncol];
+    fft_reg<T, NZ, R1, R2, false>(v);
+    const double mc = -L->ldc[b];
+    const double* G = L->G;
+    const int ky = c / nxh, kx = c - ky * nxh;
+    const double jx = (double)kx, jy = (double)fold(ky, ny);
+    const double a0 = G[0] * jx + G[1] * jy, a1 = G[3] * jx + G[4] * jy,
+                 a2 = G[6] * jx + G[7] * jy;
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) {
+        const double jz = (double)(2 * z <= NZ ? z : NZ - z);
+        const double k0 = a0 + G[2] * jz, k1 = a1 + G[5] * jz, k2 = a2 + G[8] * jz;
+        const T f = exp_t<T>(mc * (k0 * k0 + k1 * k1 + k2 * k2));
+        v[z].x *= f;
+        v[z].y *= f;
+    }
+    fft_reg<T, NZ, R1, R2, true>(v);
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) kb[(size_t)z * ncol] = v[z];
+}
+
+// ---------------------------------------------------------------------------------
 // Dispatch: returns false when the size has no specialisation (generic kernels then).
 
 template <typename T>
@@ -348,21 +466,23 @@ bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipS
 template <typename T>
 bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                 hipStream_t st) {
-#define PF_SL(X, Y)                                                                         \
+#define PF_SL1(X, Y, NT)                                                                    \
     if (nx == X && ny == Y) {                                                               \
         if (mode == SLAB_FWD)                                                               \
-            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_FWD, 256>), dim3(nl * nz), dim3(256), 0, \
+            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_FWD, NT>), dim3(nl * nz), dim3(NT), 0, \
                                st, d, nz, t);                                               \
         else                                                                                \
-            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_INVFWD, 256>), dim3(nl * nz), dim3(256), \
+            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_INVFWD, NT>), dim3(nl * nz), dim3(NT), \
                                0, st, d, nz, t);                                            \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
     }
+#define PF_SL(X, Y) PF_SL1(X, Y, 256)
     PF_SL(32, 32)
-    PF_SL(64, 64)
+    PF_SL1(64, 64, 512)
     if constexpr (sizeof(T) == 4) { PF_SL(128, 128) }
 #undef PF_SL
+#undef PF_SL1
     return false;
 }
 
@@ -376,6 +496,25 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
                            dim3(256), 0, st, d, ny, t);                                     \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
+    }
+    {
+        // one thread per column, the whole z line in registers
+        const int ncol = (nx / 2 + 1) * ny;
+#define PF_PR(Z)                                                                            \
+        if (nz == Z) {                                                                      \
+            const int per = (ncol + 63) / 64;                                               \
+            hipLaunchKernelGGL((k_pencil_reg<T, Z, 64>), dim3(nl * per), dim3(64), 0, st, d,  \
+                               ny, per, t);                                                 \
+            PF_HIP(hipGetLastError());                                                      \
+            return true;                                                                    \
+        }
+        if constexpr (sizeof(T) == 4) {
+            PF_PR(64)
+            PF_PR(32)
+        } else {
+            PF_PR(32)
+        }
+#undef PF_PR
     }
     PF_PE(32, 32, 4)
     PF_PE(64, 64, 2)

@@ -631,6 +631,12 @@ class Batch:
         call("pf_batch_timing_read", self._h, int(kind), C.byref(ms), C.byref(cnt))
         return ms.value, cnt.value
 
+    def timing_lanes(self, kind):
+        """Mean number of propagator lanes in one sampled launch of ``kind``."""
+        v = C.c_double()
+        call("pf_batch_timing_lanes", self._h, int(kind), C.byref(v))
+        return v.value
+
     def residuals_update(self):
         call("pf_batch_residuals", self._h)
 
