@@ -283,7 +283,6 @@ def main():
     if not args.no_single:
         line["single_instance_steps_per_s"] = single_instance(P, args, nsteps_mde)
     if not args.no_sweep:
-
         line["batched_scft"] = batched_scft(P, args.sweep_instances, args.plain_sample)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.seed)
