@@ -207,6 +207,10 @@ int pf_batch_destroy(pf_batch* b);
 int pf_batch_density(pf_batch* b);
 int pf_batch_set_precision(pf_batch* b, int precision);
 int pf_batch_residuals(pf_batch* b);
+/* Graph replay of the density!/residuals! launch sequence (default on): the launches of
+   every lane group, Q, Simpson, pooling and residuals are captured once per active set
+   and replayed; off runs them eagerly. */
+int pf_batch_set_graph(pf_batch* b, int on);
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask /* nsys flags or NULL: all */,
                   pf_scft_results* results);

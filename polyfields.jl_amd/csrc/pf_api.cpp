@@ -513,6 +513,14 @@ int pf_batch_density(pf_batch* b) {
     });
 }
 
+int pf_batch_set_graph(pf_batch* b, int on) {
+    return guard([&] {
+        PF_HIP(hipStreamSynchronize(B(b).stream));
+        B(b).use_graph = on != 0;
+        B(b).drop_graphs();
+    });
+}
+
 int pf_batch_set_precision(pf_batch* b, int precision) {
     return guard([&] { B(b).set_precision(precision); });
 }

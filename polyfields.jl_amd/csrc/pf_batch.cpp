@@ -34,6 +34,7 @@ Batch::Batch(const std::vector<System*>& s) : sys(s) {
 Batch::~Batch() {
     (void)hipSetDevice(sys[0]->device);
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     eng.reset();
     dev_free(sums);
     dev_free(d_step);
@@ -43,9 +44,17 @@ Batch::~Batch() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+void Batch::drop_graphs() {
+    for (auto& g : graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = Graph{};
+    }
+}
+
 void Batch::set_precision(int p) {
     if (p == precision) return;
     PF_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
     eng.reset();
     for (System* s : sys) s->set_precision(p);
     precision = p;
@@ -95,20 +104,53 @@ static int max_parts(const std::vector<SysTask>& t, bool gram) {
     return m;
 }
 
-void Batch::density(const std::vector<int>& act) {
+// density! (kind 0) or residuals! (kind 1) of the active systems.  The launch sequence
+// (exp tables, every lane group on its own stream, Q, Simpson, pooling, residuals) is
+// captured once into a graph and replayed while the active set and the lane tables are
+// unchanged; the graph holds kernel nodes only (the tables it reads are uploaded before
+// capture).  Timing runs eagerly, so that events can bracket single launches.
+void Batch::run(int kind, const std::vector<int>& act) {
     bind_active(act);
-    eng->density(act);
+    eng->prepare(act);
     size_t nmax = 0;
     for (int i : act) nmax = std::max(nmax, sys[i]->n);
-    launch_muphi_pool_b(d_act, (int)act.size(), nmax, stream);
+    auto seq = [&] {
+        eng->launch(act);
+        launch_muphi_pool_b(d_act, (int)act.size(), nmax, stream);
+        if (kind == 1)
+            launch_residuals_b(d_act, (int)act.size(), max_parts(tasks, false), nmax, stream);
+    };
+    if (!use_graph || eng->timing_on()) {
+        seq();
+        return;
+    }
+    Graph& g = graphs[kind];
+    if (!g.exec || g.act != act || g.gen != eng->generation() || g.d_act != d_act) {
+        if (g.exec) PF_HIP(hipGraphExecDestroy(g.exec));
+        g = Graph{};
+        hipGraph_t graph = nullptr;
+        PF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        try {
+            seq();
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        PF_HIP(hipStreamEndCapture(stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        PF_HIP(e);
+        g.act = act;
+        g.gen = eng->generation();
+        g.d_act = d_act;
+    }
+    PF_HIP(hipGraphLaunch(g.exec, stream));
 }
 
-void Batch::residuals(const std::vector<int>& act) {
-    density(act);
-    size_t nmax = 0;
-    for (int i : act) nmax = std::max(nmax, sys[i]->n);
-    launch_residuals_b(d_act, (int)act.size(), max_parts(tasks, false), nmax, stream);
-}
+void Batch::density(const std::vector<int>& act) { run(0, act); }
+
+void Batch::residuals(const std::vector<int>& act) { run(1, act); }
 
 void Batch::sys_sums(const std::vector<int>& act, std::vector<double>& out) {
     bind_active(act);

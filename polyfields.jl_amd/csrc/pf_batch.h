@@ -59,7 +59,17 @@ class BatchEngineBase {
    public:
     virtual ~BatchEngineBase() = default;
     // density! of the systems listed in `active` (indices into the batch)
-    virtual void density(const std::vector<int>& active) = 0;
+    virtual void density(const std::vector<int>& active) {
+        prepare(active);
+        launch(active);
+    }
+    // rebuild the lane tables if the active set or a cell changed (host work, uploads)
+    virtual void prepare(const std::vector<int>& active) = 0;
+    // the launch sequence of density! for a prepared active set (stream-ordered only, so
+    // it can be captured into a graph)
+    virtual void launch(const std::vector<int>& active) = 0;
+    virtual long generation() const = 0;   // bumped by every rebuild
+    virtual bool timing_on() const = 0;
     // HIP-event timing of the propagator launches; kind: 0 slab, 1 pencil, 2 planar
     virtual void timing(int on) = 0;
     virtual double timing_ms(int kind, int* count) = 0;
@@ -87,6 +97,17 @@ class Batch {
     std::vector<int> act_cache;      // active set the compacted descriptors were built for
     SysTask* d_act = nullptr;        // descriptors of the active systems, compacted
     std::vector<std::unique_ptr<Anderson>> and_state;
+    // graphs of the density! / residuals! launch sequences, replayed while the active set
+    // and the engine's lane tables stay the same
+    bool use_graph = true;
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<int> act;
+        long gen = -1;
+        const SysTask* d_act = nullptr;
+    } graphs[2];
+    void drop_graphs();
+    void run(int kind, const std::vector<int>& act);   // 0 density, 1 residuals
 
     Batch(const std::vector<System*>& s);
     ~Batch();

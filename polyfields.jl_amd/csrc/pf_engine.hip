@@ -811,11 +811,22 @@ class BatchEngine : public BatchEngineBase {
         return tot;
     }
 
-    void density(const std::vector<int>& act) override {
+    long gen = 0;
+
+    void prepare(const std::vector<int>& act) override {
         bool stale = act != cached;
         for (size_t k = 0; !stale && k < act.size(); ++k)
             stale = B->sys[act[k]]->cell.version != cell_versions[k] || B->sys[act[k]]->dirty_species;
-        if (stale) rebuild(act);
+        if (stale) {
+            rebuild(act);
+            ++gen;
+        }
+    }
+
+    long generation() const override { return gen; }
+    bool timing_on() const override { return tim; }
+
+    void launch(const std::vector<int>&) override {
         hipStream_t st = B->stream;
         launch_lw_fill_b<T>(d_lwt, (int)lwt.size(), d_fillt, (int)fillt.size(), nmax, st);
         PF_HIP(hipEventRecord(fork, st));

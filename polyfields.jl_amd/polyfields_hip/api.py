@@ -631,6 +631,11 @@ class Batch:
         call("pf_batch_timing_read", self._h, int(kind), C.byref(ms), C.byref(cnt))
         return ms.value, cnt.value
 
+    def use_graph(self, on):
+        """Replay the density!/residuals! launch sequence as a captured graph (default)
+        or launch it eagerly."""
+        call("pf_batch_set_graph", self._h, int(bool(on)))
+
     def timing_lanes(self, kind):
         """Mean number of propagator lanes in one sampled launch of ``kind``."""
         v = C.c_double()
