@@ -61,7 +61,7 @@ __device__ void plane_to_spectrum(cpx<T>* S, cpx<T>* kb, const Lane<T>* L) {
 
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_adj_slab(const Lane<T>* __restrict__ lanes, int nz,
-                                                  int t) {
+                                                  int t, int nextB) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* S = reinterpret_cast<cpx<T>*>(smem_raw);
     const int lane = blockIdx.x / nz;
@@ -76,6 +76,19 @@ __global__ void __launch_bounds__(256) k_adj_slab(const Lane<T>* __restrict__ la
     cpx<T>* kB = L->kb2 + (size_t)z * kplane;
     int s_in;
     int b = step_block(L, NS - 1 - t, &s_in);
+    if (MODE == 2) {
+        // B = rfft(LW a_in) of step t alone (its input slice was not in memory when the
+        // previous step finished: checkpointed sweeps)
+        const cpx<T>* ai = reinterpret_cast<const cpx<T>*>(L->fwd + (size_t)(s_in - 1) * n + off);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + off);
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            const cpx<T> q = ai[i], w = lw[i];
+            S[y * ld + (i - y * M)] = {q.x * w.x, q.y * w.y};
+        }
+        plane_to_spectrum(S, kB, L);
+        return;
+    }
     if (MODE == 0) {
         // a-bar at the first slice of the reverse sweep: dF/dQ part + Simpson source
         const int s_out = s_in + L->dir;
@@ -145,6 +158,7 @@ __global__ void __launch_bounds__(256) k_adj_slab(const Lane<T>* __restrict__ la
     }
     // A = rfft(LW a-bar) of the step about to run, then B = rfft(LW a_in)
     plane_to_spectrum(S, kA, L);
+    if (!nextB) return;
     __syncthreads();
     {
         const cpx<T>* ai = reinterpret_cast<const cpx<T>*>(L->fwd + (size_t)(s_in - 1) * n + off);
@@ -251,15 +265,19 @@ static void set_lds_adj(Kern k, int bytes) {
 
 template <typename T>
 void launch_adj_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, int lds_bytes,
-                     hipStream_t st) {
+                     hipStream_t st, int nextB) {
     if (mode == 0) {
         set_lds_adj(k_adj_slab<T, 0>, lds_bytes);
         hipLaunchKernelGGL((k_adj_slab<T, 0>), dim3(nlanes * nz), dim3(256), lds_bytes, st,
-                           d_lanes, nz, t);
-    } else {
+                           d_lanes, nz, t, nextB);
+    } else if (mode == 1) {
         set_lds_adj(k_adj_slab<T, 1>, lds_bytes);
         hipLaunchKernelGGL((k_adj_slab<T, 1>), dim3(nlanes * nz), dim3(256), lds_bytes, st,
-                           d_lanes, nz, t);
+                           d_lanes, nz, t, nextB);
+    } else {
+        set_lds_adj(k_adj_slab<T, 2>, lds_bytes);
+        hipLaunchKernelGGL((k_adj_slab<T, 2>), dim3(nlanes * nz), dim3(256), lds_bytes, st,
+                           d_lanes, nz, t, 1);
     }
     PF_HIP(hipGetLastError());
 }
@@ -273,8 +291,10 @@ void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, i
     PF_HIP(hipGetLastError());
 }
 
-template void launch_adj_slab<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
-template void launch_adj_slab<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
+template void launch_adj_slab<float>(const Lane<float>*, int, int, int, int, int, hipStream_t,
+                                     int);
+template void launch_adj_slab<double>(const Lane<double>*, int, int, int, int, int, hipStream_t,
+                                      int);
 template void launch_adj_pencil<float>(const Lane<float>*, int, int, int, int, int, int,
                                        hipStream_t);
 template void launch_adj_pencil<double>(const Lane<double>*, int, int, int, int, int, int,
@@ -352,5 +372,64 @@ void launch_point_grad(const double* w, double vol, size_t n, const double* rbar
     hipLaunchKernelGGL(k_point_grad, dim3(g), dim3(256), 0, st, w, vol, n, rbar, ca, cb, dw);
     PF_HIP(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------------
+// Simpson accumulation over a segment of slices [s_lo, s_hi] (checkpointed sweeps): the
+// contour rule of src/species/multiblock.jl:182-216 applied slice by slice,
+//   acc_b += w^b_s q_s q+_s   for every block b holding slice s,
+// then normalised by ds_b / (3 Q Nref) and summed per monomer (multiblock.jl:219-228).
+template <typename T>
+__global__ void k_simpson_seg(SimpsonTask<T> tk, int s_lo, int s_hi, double* acc) {
+    const size_t n = tk.n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        for (int s = s_lo; s <= s_hi; ++s) {
+            const double p = (double)tk.q[(size_t)(s - 1) * n + i] *
+                             (double)tk.qc[(size_t)(s - 1) * n + i];
+            for (int b = 0; b < tk.nblk; ++b) {
+                const int bgn = tk.begin[b], end = tk.begin[b + 1];
+                if (s >= bgn && s <= end) acc[(size_t)b * n + i] += simpson_w(s, bgn, end) * p;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_simpson_fin(SimpsonTask<T> tk, const double* acc) {
+    const size_t n = tk.n;
+    const double QN = tk.sc->Q[tk.isp] * tk.Nref;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double acc_m[PF_MAXMON];
+        for (int m = 0; m < tk.nmon; ++m) acc_m[m] = 0.0;
+        for (int b = 0; b < tk.nblk; ++b) {
+            double rho = acc[(size_t)b * n + i] * (tk.ds[b] / 3.0);
+            rho /= QN;
+            acc_m[tk.blk_mon[b]] += rho;
+            if (tk.rho_blk) tk.rho_blk[(size_t)b * n + i] = rho;
+        }
+        for (int m = 0; m < tk.nmon; ++m) tk.rho_sp[(size_t)m * n + i] = acc_m[m];
+    }
+}
+
+template <typename T>
+void launch_simpson_seg(const SimpsonTask<T>& tk, int s_lo, int s_hi, double* acc,
+                        hipStream_t st) {
+    const int g = (int)std::min<size_t>((tk.n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_simpson_seg<T>, dim3(g), dim3(256), 0, st, tk, s_lo, s_hi, acc);
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_simpson_fin(const SimpsonTask<T>& tk, const double* acc, hipStream_t st) {
+    const int g = (int)std::min<size_t>((tk.n + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_simpson_fin<T>, dim3(g), dim3(256), 0, st, tk, acc);
+    PF_HIP(hipGetLastError());
+}
+
+template void launch_simpson_seg<float>(const SimpsonTask<float>&, int, int, double*, hipStream_t);
+template void launch_simpson_seg<double>(const SimpsonTask<double>&, int, int, double*, hipStream_t);
+template void launch_simpson_fin<float>(const SimpsonTask<float>&, const double*, hipStream_t);
+template void launch_simpson_fin<double>(const SimpsonTask<double>&, const double*, hipStream_t);
 
 }  // namespace pf

@@ -359,6 +359,17 @@ int pf_density_vjp(pf_system* sys, const double* rho_bar, const double* Q_bar, d
     });
 }
 
+int pf_system_set_checkpoint(pf_system* sys, int K) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(K >= -1, "segment length");
+        if (K != 0 && s.mde != 0) throw pf::Error(-2, "checkpointed sweeps are provided for RK2 only");
+        s.sync();
+        s.ckpt = K;
+        s.dirty_species = 1;
+    });
+}
+
 int pf_species_set_blocks(pf_system* sys, int isp, const double* N_block, int nblk) {
     return guard([&] {
         need(N_block && nblk > 0, "block lengths");

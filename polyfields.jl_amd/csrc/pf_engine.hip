@@ -44,7 +44,18 @@ class Engine : public EngineBase {
         std::vector<T*> lw;
         std::vector<T*> lw2;       // RQM4 half-step tables
         int row0 = 0;              // first stress row of this species
+        // checkpointed sweeps: q and q+ at the segment boundaries bnd[0..nseg], and the
+        // Simpson accumulators; q / qc above are then one segment (K + 1 slices) each
+        T* ckq = nullptr;
+        T* ckqc = nullptr;
+        double* acc = nullptr;     // [blk][n]
+        std::vector<int> bnd;      // 1 = bnd[0] < ... < bnd[nseg] = Ns + 1
+        int jq = -1, jc = -1;      // lanes of q and q+
     };
+    int ckK = 0;                   // segment length in slices; 0: full stacks
+    std::vector<Lane<T>> seglanes; // [lane][segment] copies with the stack shifted
+    std::vector<int> segoff;       // first seglanes entry of each lane
+    Lane<T>* d_seglanes = nullptr;
     System* S = nullptr;
     std::vector<SpBuf> sb;
     cpx<T>* tw = nullptr;
@@ -93,7 +104,14 @@ class Engine : public EngineBase {
     void free_all() {
         free_adjoint();
         lane_sp.clear();
+        dev_free(d_seglanes);
+        d_seglanes = nullptr;
+        seglanes.clear();
+        segoff.clear();
         for (auto& b : sb) {
+            dev_free(b.ckq);
+            dev_free(b.ckqc);
+            dev_free(b.acc);
             dev_free(b.q);
             dev_free(b.qc);
             for (T* p : b.lw) dev_free(p);
@@ -157,6 +175,7 @@ class Engine : public EngineBase {
         lanes.clear();
         tasks.clear();
         sb.resize(s.species.size());
+        ckK = s.ckpt;
         int row = 0;
         for (size_t i = 0; i < s.species.size(); ++i) {
             const Species& sp = s.species[i];
@@ -164,13 +183,29 @@ class Engine : public EngineBase {
             SpBuf& b = sb[i];
             b.row0 = row;
             row += sp.Ns + 1;
-            b.q = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
-            if (sp.nblk() > 1) b.qc = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
+            if (ckK) {
+                if (s.mde != 0) throw Error(-2, "checkpointed sweeps are provided for RK2 only");
+                // segments of K slices between checkpoints; every chain gets a real q+ sweep
+                const int K = ckK > 0 ? ckK : (int)std::ceil(std::sqrt((double)sp.Ns + 1.0));
+                b.bnd.clear();
+                for (int s0 = 1; s0 < sp.Ns + 1; s0 += K) b.bnd.push_back(s0);
+                b.bnd.push_back(sp.Ns + 1);
+                const int nseg = (int)b.bnd.size() - 1;
+                b.q = (T*)dev_alloc(sizeof(T) * n * (K + 1));
+                b.qc = (T*)dev_alloc(sizeof(T) * n * (K + 1));
+                b.ckq = (T*)dev_alloc(sizeof(T) * n * (nseg + 1));
+                b.ckqc = (T*)dev_alloc(sizeof(T) * n * (nseg + 1));
+                b.acc = (double*)dev_alloc(sizeof(double) * n * sp.nblk());
+            } else {
+                b.q = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
+                if (sp.nblk() > 1) b.qc = (T*)dev_alloc(sizeof(T) * n * (sp.Ns + 1));
+            }
             for (int k = 0; k < sp.nblk(); ++k) b.lw.push_back((T*)dev_alloc(sizeof(T) * n));
             if (s.mde == 1)
                 for (int k = 0; k < sp.nblk(); ++k) b.lw2.push_back((T*)dev_alloc(sizeof(T) * n));
             maxNs = std::max(maxNs, sp.Ns);
-            for (int d = 0; d < (sp.nblk() > 1 ? 2 : 1); ++d) {
+            for (int d = 0; d < (b.qc ? 2 : 1); ++d) {
+                (d == 0 ? b.jq : b.jc) = (int)lanes.size();
                 Lane<T> L{};
                 L.nx = nx;
                 L.ny = ny;
@@ -235,7 +270,129 @@ class Engine : public EngineBase {
         if (lanes.empty() || lanes_cell_version == S->cell.version) return;
         for (auto& L : lanes) std::memcpy(L.G, S->cell.G, sizeof(L.G));
         upload(d_lanes, lanes.data(), sizeof(Lane<T>) * lanes.size());
+        if (ckK) build_seglanes();
         lanes_cell_version = S->cell.version;
+    }
+
+    // --- checkpointed sweeps ------------------------------------------------------------
+    // Segment m of a chain covers slices bnd[m]..bnd[m+1]; a lane copy whose stack pointer
+    // is shifted so that slice s lands in slot s - bnd[m] of the segment buffer.
+    void build_seglanes() {
+        seglanes.clear();
+        segoff.assign(lanes.size(), 0);
+        for (size_t j = 0; j < lanes.size(); ++j) {
+            const SpBuf& b = sb[lane_sp[j]];
+            segoff[j] = (int)seglanes.size();
+            for (size_t m = 0; m + 1 < b.bnd.size(); ++m) {
+                Lane<T> L = lanes[j];
+                L.stack = L.stack - (ptrdiff_t)(b.bnd[m] - 1) * (ptrdiff_t)S->n;
+                seglanes.push_back(L);
+            }
+        }
+        dev_free(d_seglanes);
+        d_seglanes = (Lane<T>*)dev_alloc(sizeof(Lane<T>) * seglanes.size());
+        upload(d_seglanes, seglanes.data(), sizeof(Lane<T>) * seglanes.size());
+    }
+
+    // steps [t0, t1) of lane j that fill segment m
+    void seg_steps(int j, int m, int* t0, int* t1) const {
+        const SpBuf& b = sb[lane_sp[j]];
+        const int lo = b.bnd[m], hi = b.bnd[m + 1], Ns = lanes[j].nsteps;
+        if (lanes[j].dir > 0) {
+            *t0 = lo - 1;
+            *t1 = hi - 1;
+        } else {
+            *t0 = Ns + 1 - hi;
+            *t1 = Ns + 1 - lo;
+        }
+    }
+
+    // Recompute segment m of lane j from its checkpoint into the segment buffer; with
+    // `store`, write the checkpoint at the segment's far end.
+    void run_segment(int j, int m, bool store) {
+        System& s = *S;
+        const size_t n = s.n;
+        hipStream_t st = s.stream;
+        SpBuf& b = sb[lane_sp[j]];
+        const Lane<T>& L = lanes[j];
+        const int lo = b.bnd[m], hi = b.bnd[m + 1];
+        T* seg = L.stack;
+        T* ck = L.dir > 0 ? b.ckq : b.ckqc;
+        const int from = L.dir > 0 ? m : m + 1, to = L.dir > 0 ? m + 1 : m;
+        const int slot_from = L.dir > 0 ? 0 : hi - lo, slot_to = L.dir > 0 ? hi - lo : 0;
+        PF_HIP(hipMemcpyAsync(seg + (size_t)slot_from * n, ck + (size_t)from * n, sizeof(T) * n,
+                              hipMemcpyDeviceToDevice, st));
+        int t0, t1;
+        seg_steps(j, m, &t0, &t1);
+        const Lane<T>* dl = d_seglanes + segoff[j] + m;
+        if (planar) {
+            if (!fixed || !fixed_planar<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, st))
+                launch_planar_steps<T>(dl, 1, t0, t1, lds_plane, 256, st);
+        } else {
+            const int nz = s.dims[2], ny = s.dims[1];
+            if (!fixed || !fixed_slab<T>(dl, 1, s.dims[0], ny, nz, SLAB_FWD, t0, st))
+                launch_slab<T>(dl, 1, nz, SLAB_FWD, t0, lds_plane, 256, st);
+            for (int t = t0; t < t1; ++t) {
+                if (!fixed || !fixed_pencil<T>(dl, 1, s.dims[0], ny, nz, ntiles, tile, t, st))
+                    launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st);
+                if (!fixed || !fixed_slab<T>(dl, 1, s.dims[0], ny, nz, SLAB_INVFWD, t, st))
+                    launch_slab<T>(dl, 1, nz, SLAB_INVFWD, t, lds_plane, 256, st);
+            }
+        }
+        if (store)
+            PF_HIP(hipMemcpyAsync(ck + (size_t)to * n, seg + (size_t)slot_to * n, sizeof(T) * n,
+                                  hipMemcpyDeviceToDevice, st));
+    }
+
+    // density! with checkpointed sweeps: forward sweeps keep only the boundary slices;
+    // Simpson runs segment by segment on recomputed slices.
+    void density_ckpt(System& s) {
+        const size_t n = s.n;
+        hipStream_t st = s.stream;
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            const Species& sp = s.species[i];
+            if (sp.kind != 0) continue;
+            SpBuf& b = sb[i];
+            const int nseg = (int)b.bnd.size() - 1;
+            launch_fill<T>(b.ckq, T(1), n, st);
+            launch_fill<T>(b.ckqc + (size_t)nseg * n, T(1), n, st);
+            for (int m = 0; m < nseg; ++m) run_segment(b.jq, m, true);
+            for (int m = nseg - 1; m >= 0; --m) run_segment(b.jc, m, true);
+        }
+        // Q from the last checkpoint of q
+        std::vector<SimpsonTask<T>> tq = tasks;
+        for (auto& tk : tq) {
+            const SpBuf& b = sb[tk.isp];
+            tk.q = b.ckq + (ptrdiff_t)((int)b.bnd.size() - 1 - tk.Ns) * (ptrdiff_t)n;
+        }
+        auto* d = (SimpsonTask<T>*)dev_alloc(sizeof(SimpsonTask<T>) * tq.size());
+        upload(d, tq.data(), sizeof(SimpsonTask<T>) * tq.size());
+        launch_species_Q<T>(d, (int)tq.size(), n, s.partial, reduce_parts(n), st);
+        PF_HIP(hipStreamSynchronize(st));
+        dev_free(d);
+    }
+
+    void simpson_ckpt(System& s, const std::vector<double*>* blk) {
+        const size_t n = s.n;
+        hipStream_t st = s.stream;
+        for (const SimpsonTask<T>& tk0 : tasks) {
+            SpBuf& b = sb[tk0.isp];
+            const Species& sp = s.species[tk0.isp];
+            const int nseg = (int)b.bnd.size() - 1;
+            launch_fill<double>(b.acc, 0.0, n * sp.nblk(), st);
+            for (int m = 0; m < nseg; ++m) {
+                run_segment(b.jq, m, false);
+                run_segment(b.jc, m, false);
+                SimpsonTask<T> tk = tk0;
+                tk.q = seglanes[segoff[b.jq] + m].stack;
+                tk.qc = seglanes[segoff[b.jc] + m].stack;
+                const int hi = (m == nseg - 1) ? b.bnd[m + 1] : b.bnd[m + 1] - 1;
+                launch_simpson_seg<T>(tk, b.bnd[m], hi, b.acc, st);
+            }
+            SimpsonTask<T> tk = tk0;
+            tk.rho_blk = blk ? (*blk)[tk0.isp] : nullptr;
+            launch_simpson_fin<T>(tk, b.acc, st);
+        }
     }
 
     hipEvent_t ev() {
@@ -262,6 +419,7 @@ class Engine : public EngineBase {
    public:
     // --- access for the batch engine -------------------------------------------------
     void prepare() {
+        if (S->ckpt) throw Error(-2, "systems in a batch keep full propagator stacks");
         if (S->dirty_species || sb.size() != S->species.size()) build();
         for (auto& L : lanes) std::memcpy(L.G, S->cell.G, sizeof(L.G));
     }
@@ -304,7 +462,7 @@ class Engine : public EngineBase {
     }
 
     void density(System& s) override {
-        if (s.dirty_species || sb.size() != s.species.size()) build();
+        if (s.dirty_species || sb.size() != s.species.size() || ckK != s.ckpt) build();
         refresh_lanes();
         hipStream_t st = s.stream;
         const size_t n = s.n;
@@ -317,8 +475,22 @@ class Engine : public EngineBase {
                 launch_lw_table<T>(sb[i].lw[k], s.w + (size_t)slot * n, coef, n, st);
                 if (s.mde == 1) launch_lw_half_table<T>(sb[i].lw2[k], s.w + (size_t)slot * n, coef, n, st);
             }
+            if (ckK) continue;
             launch_fill<T>(sb[i].q, T(1), n, st);
             if (sb[i].qc) launch_fill<T>(sb[i].qc + (size_t)sp.Ns * n, T(1), n, st);
+        }
+        if (ckK) {
+            density_ckpt(s);
+            const int nparts = reduce_parts(n);
+            for (size_t i = 0; i < s.species.size(); ++i) {
+                const Species& sp = s.species[i];
+                if (sp.kind != 1) continue;
+                const int slot = sp.slots[0];
+                launch_point_density(s.w + (size_t)slot * n, s.mon_vol[slot], n, sp.rho_sp,
+                                     s.partial, nparts, s.sc, (int)i, st);
+            }
+            simpson_ckpt(s, nullptr);
+            return;
         }
         const int nl = (int)lanes.size();
         if (nl) {
@@ -419,6 +591,9 @@ class Engine : public EngineBase {
     }
 
     void stress_rows(System& s, std::vector<double>& rows) override {
+        if (s.ckpt)
+            throw Error(-2, "the reference stress formula needs full propagator stacks; with "
+                            "checkpointed sweeps use the free-energy gradient's cell derivative");
         if (s.dirty_species || sb.size() != s.species.size()) build();
         if (slanes_cell_version != s.cell.version || (int)slanes.size() != nrows) build_stress();
         rows.assign((size_t)nrows * PF_MAXPAR, 0.0);
@@ -443,8 +618,64 @@ class Engine : public EngineBase {
         PF_HIP(hipStreamSynchronize(st));
     }
 
+    // Reverse sweeps on checkpointed stacks: each adjoint lane walks the segments in its
+    // own order (q-bar from the top, q+-bar from the bottom); before a segment, q and q+
+    // of that segment are recomputed from their checkpoints.  At a segment edge the
+    // next step's B spectrum is made by a separate launch once its input slice exists.
+    void adjoint_ckpt(System& s) {
+        const size_t n = s.n;
+        const int ny = s.dims[1], nz = s.dims[2];
+        hipStream_t st = s.stream;
+        std::vector<Lane<T>> aseg;
+        std::vector<int> aoff(alanes.size(), 0);
+        for (size_t j = 0; j < alanes.size(); ++j) {
+            const SpBuf& b = sb[lane_sp[j]];
+            aoff[j] = (int)aseg.size();
+            for (size_t m = 0; m + 1 < b.bnd.size(); ++m) {
+                Lane<T> A = alanes[j];
+                const ptrdiff_t sh = (ptrdiff_t)(b.bnd[m] - 1) * (ptrdiff_t)n;
+                A.fwd = (A.dir > 0 ? b.q : b.qc) - sh;
+                A.other = (A.dir > 0 ? b.qc : b.q) - sh;
+                aseg.push_back(A);
+            }
+        }
+        auto* d_aseg = (Lane<T>*)dev_alloc(sizeof(Lane<T>) * aseg.size());
+        upload(d_aseg, aseg.data(), sizeof(Lane<T>) * aseg.size());
+        for (size_t i = 0; i < s.species.size(); ++i) {
+            if (s.species[i].kind != 0) continue;
+            const SpBuf& b = sb[i];
+            const int nseg = (int)b.bnd.size() - 1;
+            for (int which = 0; which < 2; ++which) {
+                const int j = which == 0 ? b.jq : b.jc;
+                const int NS = lanes[j].nsteps;
+                for (int k = 0; k < nseg; ++k) {
+                    const int m = which == 0 ? nseg - 1 - k : k;
+                    run_segment(b.jq, m, false);
+                    run_segment(b.jc, m, false);
+                    const int lo = b.bnd[m], hi = b.bnd[m + 1];
+                    const int t0 = which == 0 ? NS + 1 - hi : lo - 1;
+                    const int t1 = which == 0 ? NS + 1 - lo : hi - 1;
+                    const Lane<T>* dA = d_aseg + aoff[j] + m;
+                    launch_adj_slab<T>(dA, 1, nz, k == 0 ? 0 : 2, t0, lds_plane, st);
+                    for (int t = t0; t < t1; ++t) {
+                        launch_adj_pencil<T>(dA, 1, ny, ntiles, tile, t, 2 * lds_pencil, st);
+                        launch_adj_slab<T>(dA, 1, nz, 1, t, lds_plane, st,
+                                           (t + 1 < t1 || k == nseg - 1) ? 1 : 0);
+                    }
+                }
+            }
+        }
+        PF_HIP(hipStreamSynchronize(st));
+        dev_free(d_aseg);
+    }
+
     void block_densities(System& s, const std::vector<double*>& dst) override {
         if (tasks.empty()) return;
+        if (ckK) {
+            simpson_ckpt(s, &dst);
+            PF_HIP(hipStreamSynchronize(s.stream));
+            return;
+        }
         std::vector<SimpsonTask<T>> tk = tasks;
         for (auto& t : tk) t.rho_blk = dst[t.isp];
         auto* d = (SimpsonTask<T>*)dev_alloc(sizeof(SimpsonTask<T>) * tk.size());
@@ -517,10 +748,14 @@ class Engine : public EngineBase {
             launch_fill<double>(A.gpart, 0.0, (size_t)per * sp.nblk() * NG, st);
         }
         upload(d_alanes, alanes.data(), sizeof(Lane<T>) * nl);
-        launch_adj_slab<T>(d_alanes, nl, nz, 0, 0, lds_plane, st);
-        for (int t = 0; t < maxNs; ++t) {
-            launch_adj_pencil<T>(d_alanes, nl, ny, ntiles, tile, t, 2 * lds_pencil, st);
-            launch_adj_slab<T>(d_alanes, nl, nz, 1, t, lds_plane, st);
+        if (!ckK) {
+            launch_adj_slab<T>(d_alanes, nl, nz, 0, 0, lds_plane, st);
+            for (int t = 0; t < maxNs; ++t) {
+                launch_adj_pencil<T>(d_alanes, nl, ny, ntiles, tile, t, 2 * lds_pencil, st);
+                launch_adj_slab<T>(d_alanes, nl, nz, 1, t, lds_plane, st);
+            }
+        } else {
+            adjoint_ckpt(s);
         }
         // pool the per-lane, per-block field derivatives; <w, wbar_b> for the ds terms
         DotArgs da{};
@@ -573,6 +808,7 @@ class Engine : public EngineBase {
     }
 
     void get_propagator(System& s, int isp, int slice, int backward, double* host) override {
+        if (ckK) throw Error(-2, "propagator stacks are not kept with checkpointed sweeps");
         if (sb.size() != s.species.size() || s.species[isp].kind != 0)
             throw Error(-2, "no propagator for this species (run density first)");
         const Species& sp = s.species[isp];

@@ -52,6 +52,7 @@ struct System {
     int dim = 0;
     int precision = 0;               // 0: FP64 everywhere, 1: FP32 propagator + FP64 sums
     int mde = 0;                     // 0 RK2, 1 RQM4 (src/system/mde.jl:18-30)
+    int ckpt = 0;                    // checkpointed sweeps: segment length (-1: sqrt(Ns+1)), 0 off
     int canonical = 1;
     int compress_mode = 0;           // 0 strict, 1 compressible, 2 unconstrained
     double inv_zeta = 0.0;

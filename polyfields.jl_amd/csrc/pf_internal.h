@@ -148,9 +148,10 @@ void launch_stress_pencil(const StressLane<T>* d_lanes, int nlanes, int ny, int 
 enum SlabMode { SLAB_FWD = 0, SLAB_INVFWD = 1 };
 
 // adjoint sweeps (pf_adjoint.hip); mode 0 starts the sweep, 1 continues after step t
+// mode 2: only the B spectrum of step t; nextB = 0: mode 1 stops after the A spectrum
 template <typename T>
 void launch_adj_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, int lds_bytes,
-                     hipStream_t st);
+                     hipStream_t st, int nextB = 1);
 template <typename T>
 void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
                        int lds_bytes, hipStream_t st);
@@ -233,6 +234,13 @@ void launch_momentum(int method, double* mt, double* vt, double* step, const dou
                      double b1, double b2, size_t N, hipStream_t st);
 void launch_segsum(const double* in, int nseg, int nparts, int stride, int nvals, double* out,
                    hipStream_t st);
+
+// checkpointed Simpson (pf_adjoint.hip): tk.q / tk.qc index slices as in a full stack
+template <typename T>
+void launch_simpson_seg(const SimpsonTask<T>& tk, int s_lo, int s_hi, double* acc,
+                        hipStream_t st);
+template <typename T>
+void launch_simpson_fin(const SimpsonTask<T>& tk, const double* acc, hipStream_t st);
 
 // gradient helpers (pf_adjoint.hip)
 struct DotArgs {

@@ -61,6 +61,7 @@ SIGNATURES = {
     "pf_fhelm_grad": (I, [P, I, DP, DP, DP, I, DP, DP, DP, DP, I]),
     "pf_density_vjp": (I, [P, DP, DP, DP, DP, I, DP, I]),
     "pf_species_set_blocks": (I, [P, I, DP, I]),
+    "pf_system_set_checkpoint": (I, [P, I]),
     "pf_free_energy_bulk": (I, [P, DP]),
     "pf_scfstress": (I, [P, DP, I]),
     "pf_updater_anderson": (I, [PP, I, I, D]),

@@ -436,6 +436,11 @@ class FieldSystem:
         call("pf_system_set_precision", self._h, {"fp64": 0, "fp32": 1}[precision])
         self.precision = precision
 
+    def set_checkpoint(self, K):
+        """Checkpointed propagator sweeps: a checkpoint every K slices (-1: sqrt(Ns+1),
+        0: full stacks)."""
+        call("pf_system_set_checkpoint", self._h, int(K))
+
     def timing(self, on):
         call("pf_system_timing", self._h, int(bool(on)))
 

@@ -149,3 +149,31 @@ def test_blend_128_gradient_fd(big):
         fd = (vals[0] - vals[1]) / 2e-6
         print(f"128^3 blend dF/dtheta[{k}] = {g.dcell[k]:.10e}, FD {fd:.10e}")
         assert g.dcell[k] == pytest.approx(fd, rel=1e-5, abs=1e-10)
+
+
+def test_blend_128_checkpointed_gradient(big):
+    """sqrt(Ns) checkpointing on the 128^3 blend: the gradient (cell derivative included)
+    from recomputed segments equals the full-stack one.  Stacks held per chain: full
+    2 (Ns + 1) slices; checkpointed 2 (Ns/K + 1) + 2 (K + 1) with K = ceil(sqrt(Ns + 1))."""
+    s64, _, f = big
+    for m, w in f.items():
+        s64.fields[m] = w
+    gf = P.fhelm_grad(s64, "textbook")
+    ck = blend(P, (128, 128, 128), cell_params())
+    ck.set_checkpoint(-1)
+    for m, w in f.items():
+        ck.fields[m] = w
+    gc = P.fhelm_grad(ck, "textbook")
+    assert gc.fhelm == pytest.approx(gf.fhelm, rel=1e-12)
+    np.testing.assert_allclose(gc.dcell, gf.dcell, rtol=1e-9)
+    for m in s64.mids:
+        err = np.max(np.abs(gc.dw[m] - gf.dw[m])) / np.max(np.abs(gf.dw[m]))
+        assert err < 1e-10
+    full = sum(2 * (sp.Ns + 1) for sp in s64.species[:1]) + (s64.species[1].Ns + 1)
+    kept = 0
+    for sp in s64.species:
+        K = math.ceil(math.sqrt(sp.Ns + 1))
+        nseg = math.ceil(sp.Ns / K)
+        kept += 2 * (nseg + 1) + 2 * (K + 1)
+    print(f"128^3 blend: slices held {full} (full stacks) vs {kept} (checkpointed), "
+          f"{full * 128 ** 3 * 8 / 2 ** 30:.2f} GiB vs {kept * 128 ** 3 * 8 / 2 ** 30:.2f} GiB FP64")
