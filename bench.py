@@ -179,6 +179,21 @@ def batched_scft(P, count, plain_sample):
     }
 
 
+# HBM bytes per lane and launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# workload (64^3 FP32, 32 lanes per launch), scaled by the counters' reading for a plain
+# float4 copy of 68 MiB (FETCH_SIZE reads half the bytes copied, WRITE_SIZE all of them):
+# slab 2 * 31197.2 + 66228.7 KiB, pencil 2 * 17123.8 + 33792.0 KiB per 32-lane launch.
+TRAFFIC_PER_LANE = {"xy-slab pass": (2 * 31197.2 + 66228.7) * 1024 / 32,
+                    "z-pencil pass": (2 * 17123.8 + 33792.0) * 1024 / 32}
+
+
+def traffic_per_launch(kernel, args, slab_lanes, pen_lanes):
+    if args.precision != "fp32" or GRID != 64:
+        return None
+    lanes = slab_lanes if kernel == "xy-slab pass" else pen_lanes
+    return TRAFFIC_PER_LANE[kernel] * lanes
+
+
 def single_instance(P, args, nsteps_mde):
     s1, _ = build(P, args.precision, args.seed)
     for m, w in seeded_fields(s1.shape, args.seed).items():
@@ -231,7 +246,8 @@ def main():
         batch.density_update()
     dt = time.perf_counter() - t0
     # kernel-level figures from a separate pass: HIP events on the launching streams
-    # bracket one launch in 8 (the launches then run eagerly, not from the graph)
+    # bracket one launch in 8; the launches then run eagerly and one stream at a time, so
+    # each timed launch has the device to itself, as in a kernel trace
     batch.timing(True)
     for _ in range(max(2, args.steps // 2)):
         batch.density_update()
@@ -278,8 +294,8 @@ def main():
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None, "bytes_per_launch": dom[1],
-                     "avg_launch_us": dom[2] * 1e6},
+                     "traffic": traffic_per_launch(dom[0], args, slab_lanes, pen_lanes),
+                     "bytes_per_launch": dom[1], "avg_launch_us": dom[2] * 1e6},
         "step_roofline_frac": value * b_step / HBM_PEAK,
         "kernel_avg_us": {"slab": slab_avg * 1e6, "pencil": pen_avg * 1e6},
     }

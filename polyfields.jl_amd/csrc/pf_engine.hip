@@ -1066,8 +1066,11 @@ class BatchEngine : public BatchEngineBase {
         hipStream_t st = B->stream;
         launch_lw_fill_b<T>(d_lwt, (int)lwt.size(), d_fillt, (int)fillt.size(), nmax, st);
         PF_HIP(hipEventRecord(fork, st));
+        hipEvent_t prev = fork;
         for (auto& gr : groups) {
-            PF_HIP(hipStreamWaitEvent(gr.st, fork, 0));
+            // with timing on, the groups run one after another, so that every timed
+            // launch has the device to itself (the figure a kernel trace reports)
+            PF_HIP(hipStreamWaitEvent(gr.st, tim ? prev : fork, 0));
             const Lane<T>* dl = d_lanes + gr.off;
             if (gr.nz == 1) {
                 timed(2, gr.st, gr.count, [&] {
@@ -1092,6 +1095,7 @@ class BatchEngine : public BatchEngineBase {
             }
             PF_HIP(hipEventRecord(gr.done, gr.st));
             PF_HIP(hipStreamWaitEvent(st, gr.done, 0));
+            prev = gr.done;
         }
         const int nparts = reduce_parts(nmax);
         if (!tasks.empty()) {

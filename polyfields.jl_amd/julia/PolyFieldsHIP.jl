@@ -226,6 +226,106 @@ function scft!(sys::HIPSystem, updater::HIPUpdater; domain = nothing,
     return SCFTResults(x.time, x.nsteps, x.converged != 0, x.ferr, x.serr, x.fhelm)
 end
 
+# ---------------------------------------------------------------------------------------
+# Reverse mode: d fhelm / d(w, cell, chi, u, phi, N_blk) (pf_fhelm_grad) and the
+# ChainRulesCore rrule that exposes it to Zygote / Diffractor.
+
+const PF_FHELM = Dict(:reference => Cint(0), :textbook => Cint(1))
+
+"""
+    fhelm_grad(sys; functional = :reference) -> (fhelm, dw::Dict{Int,Array{Float64,3}},
+                                                 dcell, dchi::Matrix, du, dphi, dN)
+
+Free energy and its gradient at the current fields (`pf_fhelm_grad`).  `:reference` is
+`free_energy` as src/system/system.jl:402-433 computes it; `:textbook` uses
+mu = log(phi / Q).  `dN` lists dF/dN_blk for every block of every chain species (contour
+grid held fixed).
+"""
+function fhelm_grad(sys::HIPSystem; functional::Symbol = :reference)
+    nm = length(sys.mids)
+    n = prod(sys.dims)
+    np = length(sys.cell.params)
+    nb = sum(sp isa Multiblock ? length(sp.N_block) : 0 for sp in sys.species; init = 0)
+    f = Ref{Float64}(0.0)
+    dw = zeros(Float64, n * nm)
+    dcell = zeros(np); dchi = zeros(nm, nm); du = zeros(nm)
+    dphi = zeros(length(sys.species)); dN = zeros(max(nb, 1))
+    check(ccall((:pf_fhelm_grad, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Ref{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
+                sys.handle, PF_FHELM[functional], f, dw, dcell, np, dchi, du, dphi,
+                nb > 0 ? dN : C_NULL, nb))
+    dwd = Dict(mid => reshape(dw[(k - 1) * n + 1:k * n], sys.dims) for (k, mid) in enumerate(sys.mids))
+    return f[], dwd, dcell, dchi, du, dphi, dN[1:nb]
+end
+
+"""
+    density_vjp(sys, rho_bar::Dict, Q_bar::Vector) -> (dw, dcell, dN)
+
+Vector-Jacobian product of `density!(sys)` (`pf_density_vjp`).
+"""
+function density_vjp(sys::HIPSystem, rho_bar::Dict, Q_bar::AbstractVector)
+    nm = length(sys.mids)
+    n = prod(sys.dims)
+    rb = reduce(vcat, (vec(Float64.(rho_bar[mid])) for mid in sys.mids))
+    dw = zeros(n * nm)
+    dcell = zeros(length(sys.cell.params))
+    nb = sum(sp isa Multiblock ? length(sp.N_block) : 0 for sp in sys.species; init = 0)
+    dN = zeros(max(nb, 1))
+    check(ccall((:pf_density_vjp, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint,
+                 Ptr{Float64}, Cint),
+                sys.handle, rb, Vector{Float64}(Q_bar), dw, dcell, length(dcell),
+                nb > 0 ? dN : C_NULL, nb))
+    return Dict(mid => reshape(dw[(k - 1) * n + 1:k * n], sys.dims) for (k, mid) in enumerate(sys.mids)),
+           dcell, dN[1:nb]
+end
+
+"""
+    set_checkpoint!(sys, K)
+
+Checkpointed sweeps: a checkpoint every K contour slices (`-1`: ceil(sqrt(Ns + 1)), `0`:
+full stacks).
+"""
+set_checkpoint!(sys::HIPSystem, K::Integer) =
+    check(ccall((:pf_system_set_checkpoint, LIB), Cint, (Ptr{Cvoid}, Cint), sys.handle, K))
+
+"""
+    fhelm(sys, w::Dict, cellparams::Vector; functional = :reference)
+
+The free energy as a function of the fields and the cell parameters: writes them into
+`sys` and evaluates.  Differentiable through the rrule below.
+"""
+function fhelm(sys::HIPSystem, w::Dict, cellparams::AbstractVector; functional::Symbol = :reference)
+    for (mid, grid) in w
+        setfield!(sys, mid, Array{Float64,3}(grid))
+    end
+    p = Vector{Float64}(cellparams)
+    check(ccall((:pf_system_cell, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint),
+                sys.handle, p, length(p), 1))
+    return fhelm_grad(sys; functional = functional)[1]
+end
+
+# The rrule is defined when ChainRulesCore is loaded alongside this module.
+function __init__()
+    if isdefined(Main, :ChainRulesCore)
+        CRC = getfield(Main, :ChainRulesCore)
+        @eval function $(CRC).rrule(::typeof(fhelm), sys::HIPSystem, w::Dict,
+                                    cellparams::AbstractVector; functional::Symbol = :reference)
+            for (mid, grid) in w
+                setfield!(sys, mid, Array{Float64,3}(grid))
+            end
+            p = Vector{Float64}(cellparams)
+            check(ccall((:pf_system_cell, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint),
+                        sys.handle, p, length(p), 1))
+            F, dw, dcell = fhelm_grad(sys; functional = functional)
+            pullback(dF) = ($(CRC).NoTangent(), $(CRC).NoTangent(),
+                            Dict(mid => dF .* g for (mid, g) in dw), dF .* dcell)
+            return F, pullback
+        end
+    end
+end
+
 using Random
 
 end # module
