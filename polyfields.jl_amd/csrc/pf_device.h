@@ -5,6 +5,41 @@
 
 namespace pf {
 
+// Pointers loaded from a Lane struct are generic to the compiler, which then emits flat
+// memory instructions: they count against the LDS wait counter too, and a flat store must
+// complete before any later LDS access.  Every buffer here is device memory, so the hot
+// kernels access complex arrays through global-address-space views (element reads and
+// writes go through the scalar members).
+template <typename C>
+struct GRef {
+    __attribute__((address_space(1))) C* p;
+    __device__ __forceinline__ operator cpx<decltype(+p->x)>() const { return {p->x, p->y}; }
+    template <typename V>
+    __device__ __forceinline__ const GRef& operator=(const V& v) const {
+        p->x = v.x;
+        p->y = v.y;
+        return *this;
+    }
+};
+template <typename C>
+struct GArr {
+    __attribute__((address_space(1))) C* p;
+    __device__ __forceinline__ GRef<C> operator[](size_t i) const { return {p + i}; }
+};
+// real arrays: element reads of a global-address-space pointer
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) const T* gps(const T* p) {
+    return (__attribute__((address_space(1))) const T*)p;
+}
+template <typename U>
+__device__ __forceinline__ GArr<cpx<U>> gp(cpx<U>* p) {
+    return {(__attribute__((address_space(1))) cpx<U>*)p};
+}
+template <typename U>
+__device__ __forceinline__ GArr<const cpx<U>> gp(const cpx<U>* p) {
+    return {(__attribute__((address_space(1))) const cpx<U>*)p};
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

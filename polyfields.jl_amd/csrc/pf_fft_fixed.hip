@@ -158,9 +158,10 @@ __device__ __forceinline__ void cplane_inverse(cpx<T>* S, const cpx<T>* twx, con
 
 template <typename T, int COUNT, int NT>
 __device__ __forceinline__ void lds_copy_in(cpx<T>* dst, const cpx<T>* __restrict__ src) {
+    const auto g = gp(src);
 #pragma unroll
     for (int i0 = 0; i0 < COUNT; i0 += NT)
-        if (i0 + (int)threadIdx.x < COUNT) dst[i0 + threadIdx.x] = src[i0 + threadIdx.x];
+        if (i0 + (int)threadIdx.x < COUNT) dst[i0 + threadIdx.x] = g[i0 + threadIdx.x];
 }
 
 // packed real plane (pairs, M per row, NY rows) <-> LDS rows of stride M+1
@@ -190,13 +191,13 @@ __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ la
     int s_in;
     step_block(L, t0, &s_in);
     {
-        const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * N);
+        const auto qin = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * N));
         for_pairs<T, NX, NY, NT>([&](int i, int o) { S[o] = qin[i]; });
     }
     const double* G = L->G;
     for (int t = t0; t < tend; ++t) {
         const int b = step_block(L, t, &s_in);
-        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b]);
+        const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b]));
         __syncthreads();
         for_pairs<T, NX, NY, NT>([&](int i, int o) {
             const cpx<T> w = lw[i];
@@ -218,7 +219,7 @@ __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ la
         }
         __syncthreads();
         cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
-        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * N);
+        const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * N));
         for_pairs<T, NX, NY, NT>([&](int i, int o) {
             const cpx<T> w = lw[i];
             const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
@@ -239,7 +240,7 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
     const Lane<T>* L = lanes + lane;
     if (t >= L->nsteps) return;
     const size_t n = (size_t)PLANE * nz;
-    cpx<T>* kb = L->kbuf + (size_t)z * KPLANE;
+    const auto kb = gp(L->kbuf + (size_t)z * KPLANE);
     lds_copy_in<T, NX, NT>(TWX, L->twx);
     lds_copy_in<T, NY, NT>(TWY, L->twy);
     int s_in;
@@ -248,40 +249,54 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
         // the LW rows of this plane are loaded first, so that their latency overlaps the
         // inverse transforms
         constexpr int NP = (NX / 2) * NY, PER = (NP + NT - 1) / NT;
-        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE);
+        const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE));
         cpx<T> wr[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = k * NT + threadIdx.x;
             if (i < NP) wr[k] = lw[i];
         }
-        lds_copy_in<T, KPLANE, NT>(S, kb);
+        lds_copy_in<T, KPLANE, NT>(S, L->kbuf + (size_t)z * KPLANE);
         __syncthreads();
         cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
         const T scale = T(2.0 / (double)n);
-        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
-                                                 (size_t)z * PLANE);
+        const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
+                                                       (size_t)z * PLANE));
         const bool more = t + 1 < L->nsteps;
         int s2;
         const int b2 = more ? step_block(L, t + 1, &s2) : b;
-        const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * PLANE);
+        const auto lw2 = gp(reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * PLANE));
+        if (b2 == b) {
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int i = k * NT + threadIdx.x;
-            if (i < NP) {
-                const int o = (i / (NX / 2)) * (NX / 2 + 1) + (i % (NX / 2));
-                const cpx<T> w = wr[k];
-                const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
-                qout[i] = v;
-                const cpx<T> w2 = (b2 == b) ? w : lw2[i];
-                S[o] = {v.x * w2.x, v.y * w2.y};
+            for (int k = 0; k < PER; ++k) {
+                const int i = k * NT + threadIdx.x;
+                if (i < NP) {
+                    const int o = (i / (NX / 2)) * (NX / 2 + 1) + (i % (NX / 2));
+                    const cpx<T> w = wr[k];
+                    const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+                    qout[i] = v;
+                    S[o] = {v.x * w.x, v.y * w.y};
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int i = k * NT + threadIdx.x;
+                if (i < NP) {
+                    const int o = (i / (NX / 2)) * (NX / 2 + 1) + (i % (NX / 2));
+                    const cpx<T> w = wr[k];
+                    const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+                    qout[i] = v;
+                    const cpx<T> w2 = lw2[i];
+                    S[o] = {v.x * w2.x, v.y * w2.y};
+                }
             }
         }
         if (!more) return;
     } else {
-        const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
-                                                            (size_t)z * PLANE);
-        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE);
+        const auto qin = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
+                                                            (size_t)z * PLANE));
+        const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE));
         for_pairs<T, NX, NY, NT>([&](int i, int o) {
             const cpx<T> q = qin[i], w = lw[i];
             S[o] = {q.x * w.x, q.y * w.y};
@@ -310,7 +325,7 @@ __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ la
     lds_copy_in<T, NZ, NT>(TWZ, L->twz);
     int s_in;
     const int b = step_block(L, t, &s_in);
-    cpx<T>* kb = L->kbuf + (size_t)ky0 * NXH;
+    const auto kb = gp(L->kbuf + (size_t)ky0 * NXH);
     const size_t zstride = (size_t)ny * NXH;
 #pragma unroll
     for (int i0 = 0; i0 < NZ * C; i0 += NT) {
@@ -415,7 +430,7 @@ __global__ void __launch_bounds__(NT) k_pencil_reg(const Lane<T>* __restrict__ l
     if (c >= ncol) return;
     int s_in;
     const int b = step_block(L, t, &s_in);
-    cpx<T>* kb = L->kbuf + c;
+    const auto kb = gp(L->kbuf + c);
     cpx<T> v[NZ];
 #pragma unroll
     for (int z = 0; z < NZ; ++z) v[z] = kb[(size_t)z * ncol];

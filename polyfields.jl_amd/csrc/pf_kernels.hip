@@ -4,7 +4,7 @@
 //
 // Reductions are two-stage and deterministic: each workgroup writes one partial row
 // (fixed grid-stride order), then one workgroup sums the rows in index order.
-#include "pf_internal.h"
+#include "pf_device.h"
 
 namespace pf {
 
@@ -145,10 +145,13 @@ __global__ void k_simpson(const SimpsonTask<T>* __restrict__ tasks) {
         for (int m = 0; m < tk->nmon; ++m) acc_m[m] = 0.0;
         for (int b = 0; b < tk->nblk; ++b) {
             const int bgn = tk->begin[b], end = tk->begin[b + 1];
-            auto qv = [&](int s) { return (double)tk->q[(size_t)(s - 1) * n + i]; };
+            const auto q = gps(tk->q);
+            const auto qc = gps(tk->qc ? tk->qc : tk->q);
+            const bool rev = tk->qc == nullptr;
+            auto qv = [&](int s) { return (double)q[(size_t)(s - 1) * n + i]; };
             auto qcv = [&](int s) {
-                return tk->qc ? (double)tk->qc[(size_t)(s - 1) * n + i]
-                              : (double)tk->q[(size_t)(Ns + 1 - s) * n + i];
+                return rev ? (double)qc[(size_t)(Ns + 1 - s) * n + i]
+                           : (double)qc[(size_t)(s - 1) * n + i];
             };
             double rho = qv(bgn) * qcv(bgn);
             rho += qv(end) * qcv(end);
