@@ -30,8 +30,12 @@ constexpr CPlan cplan(int n) {
 
 // One Stockham pass (compile-time).  NL lines of length N, element e of line l at
 // buf[l*LS + e*ES]; NT threads; twiddle w_N^m = tw[m*TWS] (LDS).
-template <typename T, int N, int R, int NS, int NL, int ES, int LS, int NT, bool INV, int TWS>
-__device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __restrict__ tw) {
+// src == dst: in place (all reads of a round finish before its writes); otherwise the
+// pass reads src and writes dst and needs one barrier instead of two.
+template <typename T, int N, int R, int NS, int NL, int ES, int LS, int NT, bool INV, int TWS,
+          bool PP = false>
+__device__ __forceinline__ void cpass(const cpx<T>* src_buf, cpx<T>* dst_buf,
+                                      const cpx<T>* __restrict__ tw) {
     constexpr int NB = N / R;
     constexpr int BPR = (R == 8) ? 2 : (R == 2 ? 8 : 4);
     static_assert(NB <= BPR * NT, "transform too long for the workgroup");
@@ -52,7 +56,7 @@ __device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __
             if (i * NT < total && b < total) {
                 const int line = LINE_FAST ? l0 + b % nlr : l0 + b / NB;
                 const int j = LINE_FAST ? b / nlr : b % NB;
-                const cpx<T>* src = buf + line * LS + j * ES;
+                const cpx<T>* src = src_buf + line * LS + j * ES;
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[i][r] = src[r * NB * ES];
                 if constexpr (NS > 1) {
@@ -66,7 +70,7 @@ __device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __
                 }
             }
         }
-        __syncthreads();
+        if constexpr (!PP) __syncthreads();
 #pragma unroll
         for (int i = 0; i < BPR; ++i) {
             const int b = threadIdx.x + i * NT;
@@ -75,7 +79,7 @@ __device__ __forceinline__ void cpass(cpx<T>* __restrict__ buf, const cpx<T>* __
                 const int j = LINE_FAST ? b / nlr : b % NB;
                 dft<T, R, INV>(v[i]);
                 const int d = (j / NS) * NS * R + (j % NS);
-                cpx<T>* dst = buf + line * LS + d * ES;
+                cpx<T>* dst = dst_buf + line * LS + d * ES;
 #pragma unroll
                 for (int r = 0; r < R; ++r) dst[r * NS * ES] = v[i][r];
             }
@@ -88,8 +92,22 @@ template <typename T, int N, int NL, int ES, int LS, int NT, bool INV, int TWS, 
 __device__ __forceinline__ void cfft(cpx<T>* buf, const cpx<T>* tw) {
     constexpr CPlan P = cplan(N);
     if constexpr (IP < P.npass) {
-        cpass<T, N, P.r[IP], P.ns[IP], NL, ES, LS, NT, INV, TWS>(buf, tw);
+        cpass<T, N, P.r[IP], P.ns[IP], NL, ES, LS, NT, INV, TWS>(buf, buf, tw);
         cfft<T, N, NL, ES, LS, NT, INV, TWS, IP + 1>(buf, tw);
+    }
+}
+
+// Ping-pong between buf and tmp (an even pass count leaves the result in buf).
+template <typename T, int N, int NL, int ES, int LS, int NT, bool INV, int TWS, int IP = 0>
+__device__ __forceinline__ void cfft_pp(cpx<T>* buf, cpx<T>* tmp, const cpx<T>* tw) {
+    constexpr CPlan P = cplan(N);
+    static_assert(P.npass % 2 == 0, "ping-pong transforms need an even number of passes");
+    if constexpr (IP < P.npass) {
+        if constexpr (IP % 2 == 0)
+            cpass<T, N, P.r[IP], P.ns[IP], NL, ES, LS, NT, INV, TWS, true>(buf, tmp, tw);
+        else
+            cpass<T, N, P.r[IP], P.ns[IP], NL, ES, LS, NT, INV, TWS, true>(tmp, buf, tw);
+        cfft_pp<T, N, NL, ES, LS, NT, INV, TWS, IP + 1>(buf, tmp, tw);
     }
 }
 
@@ -146,6 +164,24 @@ __device__ __forceinline__ void cplane_forward(cpx<T>* S, const cpx<T>* twx, con
     cfft<T, M, NY, 1, LD, NT, false, 2>(S, twx);
     csplit_r2c<T, M, NY, LD, NT>(S, twx);
     if constexpr (NY > 1) cfft<T, NY, M + 1, LD, 1, NT, false, 1>(S, twy);
+}
+
+template <typename T, int NX, int NY, int NT>
+__device__ __forceinline__ void cplane_forward_pp(cpx<T>* S, cpx<T>* S2, const cpx<T>* twx,
+                                                  const cpx<T>* twy) {
+    constexpr int M = NX / 2, LD = M + 1;
+    cfft_pp<T, M, NY, 1, LD, NT, false, 2>(S, S2, twx);
+    csplit_r2c<T, M, NY, LD, NT>(S, twx);
+    cfft_pp<T, NY, M + 1, LD, 1, NT, false, 1>(S, S2, twy);
+}
+
+template <typename T, int NX, int NY, int NT>
+__device__ __forceinline__ void cplane_inverse_pp(cpx<T>* S, cpx<T>* S2, const cpx<T>* twx,
+                                                  const cpx<T>* twy) {
+    constexpr int M = NX / 2, LD = M + 1;
+    cfft_pp<T, NY, M + 1, LD, 1, NT, true, 1>(S, S2, twy);
+    csplit_c2r<T, M, NY, LD, NT>(S, twx);
+    cfft_pp<T, M, NY, 1, LD, NT, true, 2>(S, S2, twx);
 }
 
 template <typename T, int NX, int NY, int NT>
@@ -232,7 +268,11 @@ __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ la
 template <typename T, int NX, int NY, int MODE, int NT>
 __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
     constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
+    // ping-pong transforms (one barrier per pass) where a second plane of LDS is cheap
+    constexpr bool PPS = sizeof(T) == 4 && NX <= 64 && NY > 1 && cplan(NX / 2).npass % 2 == 0 &&
+                         cplan(NY).npass % 2 == 0;
     __shared__ cpx<T> S[KPLANE];
+    __shared__ cpx<T> S2[PPS ? KPLANE : 1];
     __shared__ cpx<T> TWX[NX];
     __shared__ cpx<T> TWY[NY];
     const int lane = blockIdx.x / nz;
@@ -258,7 +298,10 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
         }
         lds_copy_in<T, KPLANE, NT>(S, L->kbuf + (size_t)z * KPLANE);
         __syncthreads();
-        cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
+        if constexpr (PPS)
+            cplane_inverse_pp<T, NX, NY, NT>(S, S2, TWX, TWY);
+        else
+            cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
         const T scale = T(2.0 / (double)n);
         const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
                                                        (size_t)z * PLANE));
@@ -303,7 +346,10 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
         });
     }
     __syncthreads();
-    cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+    if constexpr (PPS)
+        cplane_forward_pp<T, NX, NY, NT>(S, S2, TWX, TWY);
+    else
+        cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
 #pragma unroll
     for (int i0 = 0; i0 < KPLANE; i0 += NT)
         if (i0 + (int)threadIdx.x < KPLANE) kb[i0 + threadIdx.x] = S[i0 + threadIdx.x];
