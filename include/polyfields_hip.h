@@ -219,6 +219,21 @@ int pf_batch_residuals(pf_batch* b);
    every lane group, Q, Simpson, pooling and residuals are captured once per active set
    and replayed; off runs them eagerly. */
 int pf_batch_set_graph(pf_batch* b, int on);
+
+/* Autotuner (no counterpart in the reference).  Times every candidate kernel schedule of
+   the grid on the caller's own workload, `reps` density! evaluations each, and keeps the
+   fastest for this grid size and precision for the rest of the process.  Candidates:
+   3-D xy-slab form (0 generic, 1 compile-time 256 threads, 2 compile-time 512 threads) x
+   z-pencil form (0 generic, 1 compile-time LDS, 2 register-resident) x, in a batch, the
+   streams per lane group (1 or 2); 1-D/2-D planar kernel (0 generic, 1 compile-time).
+   Schedules with different transform forms round differently (about 1 ulp per step).
+   sched receives {slab, pencil, planar, split} (-1: not a candidate for this grid). */
+int pf_system_autotune(pf_system* sys, int reps, int* sched, double* best_ms);
+/* Per distinct grid size of the batch: sched[8k .. 8k+7] = {bytes, nx, ny, nz, slab,
+   pencil, planar, split} for k < min(cap, *count). */
+int pf_batch_autotune(pf_batch* b, int reps, int* sched, int cap, int* count);
+/* Forget every tuned schedule (built-in defaults again). */
+int pf_schedule_reset(void);
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask /* nsys flags or NULL: all */,
                   pf_scft_results* results);

@@ -524,6 +524,40 @@ int pf_batch_density(pf_batch* b) {
     });
 }
 
+int pf_system_autotune(pf_system* sys, int reps, int* sched, double* best_ms) {
+    return guard([&] {
+        need(reps > 0, "repetitions");
+        const pf::Schedule s = pf::autotune_system(S(sys), reps, best_ms);
+        if (sched) {
+            sched[0] = s.slab;
+            sched[1] = s.pencil;
+            sched[2] = s.planar;
+            sched[3] = s.split;
+        }
+    });
+}
+
+int pf_schedule_reset(void) {
+    return guard([&] { pf::schedule_reset(); });
+}
+
+int pf_batch_autotune(pf_batch* b, int reps, int* sched, int cap, int* count) {
+    return guard([&] {
+        need(reps > 0, "repetitions");
+        std::vector<std::pair<std::array<int, 4>, pf::Schedule>> out;
+        pf::autotune_batch(B(b), reps, out);
+        if (count) *count = (int)out.size();
+        for (int k = 0; sched && k < (int)out.size() && k < cap; ++k) {
+            int* r = sched + 8 * k;
+            for (int j = 0; j < 4; ++j) r[j] = out[k].first[j];
+            r[4] = out[k].second.slab;
+            r[5] = out[k].second.pencil;
+            r[6] = out[k].second.planar;
+            r[7] = out[k].second.split;
+        }
+    });
+}
+
 int pf_batch_set_graph(pf_batch* b, int on) {
     return guard([&] {
         PF_HIP(hipStreamSynchronize(B(b).stream));

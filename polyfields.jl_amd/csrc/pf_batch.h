@@ -2,6 +2,8 @@
 // of all systems, and every system-level kernel covers all systems (blockIdx.y).
 #pragma once
 
+#include <array>
+
 #include "pf_host.h"
 
 namespace pf {
@@ -69,6 +71,7 @@ class BatchEngineBase {
     // it can be captured into a graph)
     virtual void launch(const std::vector<int>& active) = 0;
     virtual long generation() const = 0;   // bumped by every rebuild
+    virtual void invalidate() = 0;          // force a rebuild (schedule change)
     virtual bool timing_on() const = 0;
     // HIP-event timing of the propagator launches; kind: 0 slab, 1 pencil, 2 planar
     virtual void timing(int on) = 0;
@@ -121,5 +124,10 @@ class Batch {
 };
 
 std::unique_ptr<BatchEngineBase> make_batch_engine(Batch& b);
+
+// autotuner (pf_tune.cpp)
+Schedule autotune_system(System& s, int reps, double* best_ms);
+void autotune_batch(Batch& b, int reps,
+                    std::vector<std::pair<std::array<int, 4>, Schedule>>& out);
 
 }  // namespace pf

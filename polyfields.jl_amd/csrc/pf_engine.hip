@@ -853,8 +853,8 @@ class BatchEngine : public BatchEngineBase {
     std::vector<long> cell_versions;
     hipEvent_t fork = nullptr;
     std::vector<std::pair<hipStream_t, hipEvent_t>> streams;   // one per (sub-)group
-    // 3-D groups of at least 4 lanes run as 2 sub-groups on 2 streams: measured on MI355X
-    // (64^3 FP32, 32 systems) 2 streams give +20% over 1, 3 streams no further gain
+    // 3-D groups of at least 4 lanes run as 2 sub-groups on 2 streams by default: measured
+    // on MI355X (64^3 FP32, 32 systems) 2 streams give +20% over 1, 3 no further gain
     static constexpr int split = 2;
     size_t nmax = 0;
     bool tim = false;
@@ -952,7 +952,9 @@ class BatchEngine : public BatchEngineBase {
         std::vector<Group> fin;
         for (size_t k = 0; k < g.size(); ++k) {
             const int cnt = (int)by_size[k].size();
-            const int parts = (g[k].nz > 1) ? std::max(1, std::min(split, cnt / 2)) : 1;
+            const int sp = schedule(sizeof(T), g[k].nx, g[k].ny, g[k].nz).split;
+            const int want = sp > 0 ? sp : split;
+            const int parts = (g[k].nz > 1) ? std::max(1, std::min(want, cnt / 2)) : 1;
             int done = 0;
             for (int p = 0; p < parts; ++p) {
                 const int c = (cnt - done) / (parts - p);
@@ -1060,6 +1062,7 @@ class BatchEngine : public BatchEngineBase {
     }
 
     long generation() const override { return gen; }
+    void invalidate() override { cached.clear(); }
     bool timing_on() const override { return tim; }
 
     void launch(const std::vector<int>&) override {

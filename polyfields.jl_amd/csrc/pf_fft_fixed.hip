@@ -8,6 +8,10 @@
 //     in every pass;
 //   * radix-8 passes (64 = 8*8, 128 = 8*8*2, 96 = 8*4*3) to cut the number of LDS
 //     round trips and workgroup barriers.
+#include <array>
+#include <map>
+#include <mutex>
+
 #include "pf_device.h"
 
 namespace pf {
@@ -501,10 +505,25 @@ __global__ void __launch_bounds__(NT) k_pencil_reg(const Lane<T>* __restrict__ l
 }
 
 // ---------------------------------------------------------------------------------
-// Dispatch: returns false when the size has no specialisation (generic kernels then).
+// Schedules and dispatch: the dispatchers return false when the schedule asks for the
+// generic kernels or the size has no specialisation.
+
+static std::mutex sched_mu;
+static std::map<std::array<int, 4>, Schedule> sched_table;
+
+Schedule& schedule(int elem_bytes, int nx, int ny, int nz) {
+    std::lock_guard<std::mutex> lock(sched_mu);
+    return sched_table[{elem_bytes, nx, ny, nz}];
+}
+
+void schedule_reset() {
+    std::lock_guard<std::mutex> lock(sched_mu);
+    sched_table.clear();
+}
 
 template <typename T>
 bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st) {
+    if (schedule(sizeof(T), nx, ny, 1).planar == 0) return false;
 #define PF_PL(X, Y, NT)                                                                    \
     if (nx == X && ny == Y) {                                                              \
         hipLaunchKernelGGL((k_planar_fx<T, X, Y, NT>), dim3(nl), dim3(NT), 0, st, d, t0, t1); \
@@ -527,6 +546,8 @@ bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipS
 template <typename T>
 bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                 hipStream_t st) {
+    const int sk = schedule(sizeof(T), nx, ny, nz).slab;
+    if (sk == 0) return false;
 #define PF_SL1(X, Y, NT)                                                                    \
     if (nx == X && ny == Y) {                                                               \
         if (mode == SLAB_FWD)                                                               \
@@ -540,6 +561,7 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
     }
 #define PF_SL(X, Y) PF_SL1(X, Y, 256)
     PF_SL(32, 32)
+    if (sk == 1) { PF_SL1(64, 64, 256) }
     PF_SL1(64, 64, 512)
     if constexpr (sizeof(T) == 4) { PF_SL(128, 128) }
 #undef PF_SL
@@ -551,6 +573,8 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
 template <typename T>
 bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, int t,
                   hipStream_t st) {
+    const int pk = schedule(sizeof(T), nx, ny, nz).pencil;
+    if (pk == 0) return false;
 #define PF_PE(X, Z, KB)                                                                     \
     if (nx == X && nz == Z && ny % KB == 0) {                                               \
         hipLaunchKernelGGL((k_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(nl * (ny / KB)),    \
@@ -558,7 +582,7 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
     }
-    {
+    if (pk != 1) {
         // one thread per column, the whole z line in registers
         const int ncol = (nx / 2 + 1) * ny;
 #define PF_PR(Z)                                                                            \

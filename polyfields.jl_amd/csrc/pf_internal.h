@@ -156,6 +156,17 @@ template <typename T>
 void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
                        int lds_bytes, hipStream_t st);
 
+// Kernel schedule of one grid size and precision, process-wide (pf_fft_fixed.hip).  -1
+// keeps the built-in default; pf_autotune times the candidates and writes the fastest.
+struct Schedule {
+    int slab = -1;     // 0 generic, 1 compile-time kernel with 256 threads, 2 with 512
+    int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil
+    int planar = -1;   // 0 generic, 1 compile-time planar kernel
+    int split = -1;    // streams per 3-D lane group in a batch (1 or 2)
+};
+Schedule& schedule(int elem_bytes, int nx, int ny, int nz);
+void schedule_reset();   // every grid back to the built-in defaults
+
 // compile-time specialised variants (pf_fft_fixed.hip); false: size not specialised
 template <typename T>
 bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st);

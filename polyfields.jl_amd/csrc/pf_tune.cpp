@@ -1,0 +1,107 @@
+// Autotuner: times the candidate kernel schedules of a grid size on the caller's own
+// workload (one density! per repetition) and keeps the fastest in the process-wide
+// schedule table that the launchers consult.  Candidates: for 3-D grids the xy-slab form
+// (generic, compile-time with 256 or 512 threads) x the z-pencil form (generic,
+// compile-time LDS, register-resident), and in batches the number of streams per lane
+// group; for 1-D/2-D grids the generic or compile-time planar kernel.
+#include <algorithm>
+#include <functional>
+#include <limits>
+
+#include "pf_batch.h"
+
+namespace pf {
+
+static double time_ms(hipStream_t st, const std::function<void()>& run, int reps) {
+    hipEvent_t a, b;
+    PF_HIP(hipEventCreate(&a));
+    PF_HIP(hipEventCreate(&b));
+    run();                                       // warm-up (and any rebuild)
+    PF_HIP(hipStreamSynchronize(st));
+    PF_HIP(hipEventRecord(a, st));
+    for (int r = 0; r < reps; ++r) run();
+    PF_HIP(hipEventRecord(b, st));
+    PF_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms / reps;
+}
+
+static std::vector<Schedule> candidates(int nz, bool batch) {
+    std::vector<Schedule> out;
+    if (nz > 1) {
+        for (int sl : {0, 1, 2})
+            for (int pe : {0, 1, 2})
+                for (int sp : batch ? std::vector<int>{1, 2} : std::vector<int>{-1}) {
+                    Schedule c;
+                    c.slab = sl;
+                    c.pencil = pe;
+                    c.split = sp;
+                    out.push_back(c);
+                }
+    } else {
+        for (int pl : {0, 1}) {
+            Schedule c;
+            c.planar = pl;
+            out.push_back(c);
+        }
+    }
+    return out;
+}
+
+// Tune the schedule of (elem, dims) with `run` timing one density!; returns the best.
+static Schedule tune(int elem, const int* dims, hipStream_t st, bool batch,
+                     const std::function<void()>& run, const std::function<void()>& changed,
+                     int reps, double* best_ms) {
+    Schedule& sc = schedule(elem, dims[0], dims[1], dims[2]);
+    Schedule best = sc;
+    double tbest = std::numeric_limits<double>::infinity();
+    for (const Schedule& c : candidates(dims[2], batch)) {
+        sc = c;
+        changed();
+        const double t = time_ms(st, run, reps);
+        if (t < tbest) {
+            tbest = t;
+            best = c;
+        }
+    }
+    sc = best;
+    changed();
+    if (best_ms) *best_ms = tbest;
+    return best;
+}
+
+Schedule autotune_system(System& s, int reps, double* best_ms) {
+    const int elem = s.precision ? 4 : 8;
+    return tune(elem, s.dims, s.stream, false, [&] { s.density(); }, [] {}, reps, best_ms);
+}
+
+void autotune_batch(Batch& b, int reps, std::vector<std::pair<std::array<int, 4>, Schedule>>& out) {
+    std::vector<int> all(b.sys.size());
+    for (size_t i = 0; i < all.size(); ++i) all[i] = (int)i;
+    const int elem = b.precision ? 4 : 8;
+    std::vector<std::array<int, 3>> sizes;
+    for (System* s : b.sys) {
+        std::array<int, 3> d = {s->dims[0], s->dims[1], s->dims[2]};
+        if (std::find(sizes.begin(), sizes.end(), d) == sizes.end()) sizes.push_back(d);
+    }
+    const bool graph = b.use_graph;
+    b.use_graph = false;
+    auto changed = [&] {
+        PF_HIP(hipStreamSynchronize(b.stream));
+        b.drop_graphs();
+        b.eng->invalidate();
+    };
+    for (const auto& d : sizes) {
+        double ms = 0.0;
+        const Schedule s = tune(elem, d.data(), b.stream, true, [&] { b.density(all); }, changed,
+                                reps, &ms);
+        out.push_back({{elem, d[0], d[1], d[2]}, s});
+    }
+    b.use_graph = graph;
+    changed();
+}
+
+}  // namespace pf

@@ -85,6 +85,9 @@ SIGNATURES = {
     "pf_batch_scft": (I, [P, I, I, D, I, D, IP, C.POINTER(ScftResults)]),
     "pf_batch_timing": (I, [P, I]),
     "pf_batch_set_graph": (I, [P, I]),
+    "pf_system_autotune": (I, [P, I, IP, DP]),
+    "pf_batch_autotune": (I, [P, I, IP, I, IP]),
+    "pf_schedule_reset": (I, []),
     "pf_batch_timing_lanes": (I, [P, I, DP]),
     "pf_batch_timing_read": (I, [P, I, DP, IP]),
     "pf_system_timing": (I, [P, I]),

@@ -436,6 +436,15 @@ class FieldSystem:
         call("pf_system_set_precision", self._h, {"fp64": 0, "fp32": 1}[precision])
         self.precision = precision
 
+    def autotune(self, reps=3):
+        """Time the candidate kernel schedules of this grid on this system and keep the
+        fastest (process-wide); returns the chosen schedule and its time per density!."""
+        sched = np.zeros(4, dtype=np.int32)
+        ms = C.c_double()
+        call("pf_system_autotune", self._h, int(reps), iptr(sched), C.byref(ms))
+        keys = ("slab", "pencil", "planar", "split")
+        return {k: int(v) for k, v in zip(keys, sched)}, ms.value
+
     def set_checkpoint(self, K):
         """Checkpointed propagator sweeps: a checkpoint every K slices (-1: sqrt(Ns+1),
         0: full stacks)."""
@@ -636,6 +645,14 @@ class Batch:
         call("pf_batch_timing_read", self._h, int(kind), C.byref(ms), C.byref(cnt))
         return ms.value, cnt.value
 
+    def autotune(self, reps=3):
+        """Tune the kernel schedule of every grid size in the batch (process-wide)."""
+        out = np.zeros(8 * 16, dtype=np.int32)
+        cnt = C.c_int()
+        call("pf_batch_autotune", self._h, int(reps), iptr(out), 16, C.byref(cnt))
+        keys = ("bytes", "nx", "ny", "nz", "slab", "pencil", "planar", "split")
+        return [{k: int(v) for k, v in zip(keys, out[8 * i:8 * i + 8])} for i in range(cnt.value)]
+
     def use_graph(self, on):
         """Replay the density!/residuals! launch sequence as a captured graph (default)
         or launch it eagerly."""
@@ -677,3 +694,8 @@ class Batch:
             except Exception:
                 pass
             self._h = None
+
+
+def schedule_reset():
+    """Forget every tuned kernel schedule (built-in defaults again)."""
+    call("pf_schedule_reset")

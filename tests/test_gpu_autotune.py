@@ -1,0 +1,71 @@
+"""Autotuner (pf_system_autotune / pf_batch_autotune): the chosen schedule is one of the
+candidates, every candidate computes the same density! (to the rounding of the transform
+form), and the tuned choice then serves later calls.  Schedules are reset at the end so
+that later tests run the built-in defaults."""
+import numpy as np
+import pytest
+
+from helpers import diblock_system
+
+pytestmark = pytest.mark.gpu
+
+P = pytest.importorskip("polyfields_hip")
+
+
+def seeded(s, seed=1):
+    rng = np.random.default_rng(seed)
+    for m in s.mids:
+        s.fields[m] = 0.3 * rng.standard_normal(s.shape)
+
+
+@pytest.fixture(autouse=True)
+def reset_after():
+    yield
+    P.schedule_reset()
+
+
+@pytest.mark.parametrize("dims,dim,crystal,precision,tol", [
+    ((32, 32, 32), 3, "cubic", "fp64", 1e-12),
+    ((64, 64, 64), 3, "cubic", "fp32", 2e-5),
+    ((48, 48, 1), 2, "hexagonal", "fp64", 1e-12),
+])
+def test_system_autotune(dims, dim, crystal, precision, tol):
+    s, _, _ = diblock_system(P, dims, dim, crystal, (5.0,), N=100, f=0.35, chi=0.2, Ns=20,
+                             precision=precision)
+    seeded(s)
+    s.density_update()
+    ref = {m: s.density[m] for m in s.mids}
+    sched, ms = s.autotune(reps=2)
+    assert ms > 0
+    if dims[2] > 1:
+        assert sched["slab"] in (0, 1, 2) and sched["pencil"] in (0, 1, 2)
+    else:
+        assert sched["planar"] in (0, 1)
+    seeded(s)
+    s.density_update()
+    for m in s.mids:
+        err = np.max(np.abs(s.density[m] - ref[m])) / np.max(np.abs(ref[m]))
+        assert err < tol
+
+
+def test_batch_autotune_then_scft():
+    syss = []
+    for dims, dim, crystal in [((32, 32, 1), 2, "hexagonal"), ((16, 16, 16), 3, "cubic"),
+                               ((16, 16, 16), 3, "cubic")]:
+        s, _, _ = diblock_system(P, dims, dim, crystal, (4.5,), N=100, f=0.35, chi=0.2, Ns=20)
+        seeded(s, seed=len(syss))
+        syss.append(s)
+    batch = P.Batch(syss)
+    batch.residuals_update()
+    ref = [s.field_error() for s in syss]
+    out = batch.autotune(reps=2)
+    assert len(out) == 2
+    for r in out:
+        assert r["bytes"] == 8
+        if r["nz"] > 1:
+            assert r["split"] in (1, 2)
+    for k, s in enumerate(syss):
+        seeded(s, seed=k)
+    batch.residuals_update()
+    for a, s in zip(ref, syss):
+        assert s.field_error() == pytest.approx(a, rel=1e-10)
