@@ -25,8 +25,9 @@ __device__ void fft_pass(cpx<T>* buf, int N, int Ns, int nlines, int es, int ls,
     const int nthr = blockDim.x;
     const int lpr = max(1, (nthr * MAXB) / nb);
     const int twmul = (N / (Ns * R)) * twstride;
-    // strided lines (es > 1, ls == 1): consecutive threads on consecutive lines
-    const bool lf = (es > 1 && ls == 1);
+    // consecutive threads on consecutive lines (unit or odd line stride: minimum number
+    // of hits per LDS bank)
+    const bool lf = (ls == 1 || (ls & 1)) && nlines > 1;
     for (int l0 = 0; l0 < nlines; l0 += lpr) {
         const int nlr = min(lpr, nlines - l0);
         const int total = nlr * nb;
@@ -95,7 +96,7 @@ __device__ void rows_r2c_split(cpx<T>* S, int M, int nrows, int ld,
                                const cpx<T>* __restrict__ twN) {
     const int half = M / 2 + 1;
     for (int i = threadIdx.x; i < nrows * half; i += blockDim.x) {
-        const int y = i / half, k = i - y * half;
+        const int k = i / nrows, y = i - k * nrows;
         cpx<T>* row = S + y * ld;
         const int km = (M - k) % M;
         const cpx<T> zk = row[k], zm = row[km];
@@ -116,7 +117,7 @@ __device__ void rows_c2r_split(cpx<T>* S, int M, int nrows, int ld,
                                const cpx<T>* __restrict__ twN) {
     const int half = M / 2 + 1;
     for (int i = threadIdx.x; i < nrows * half; i += blockDim.x) {
-        const int y = i / half, k = i - y * half;
+        const int k = i / nrows, y = i - k * nrows;
         cpx<T>* row = S + y * ld;
         cpx<T> xa = row[k], xb = row[M - k];
         if (k == 0) {

@@ -46,9 +46,11 @@ __device__ __forceinline__ void cpass(const cpx<T>* src_buf, cpx<T>* dst_buf,
     constexpr int LPR0 = (BPR * NT) / NB;
     constexpr int LPR = LPR0 < 1 ? 1 : (LPR0 > NL ? NL : LPR0);
     constexpr int TWM = (N / (NS * R)) * TWS;
-    // Strided lines (ES > 1, LS == 1): consecutive threads take consecutive lines, so a
-    // wavefront touches consecutive LDS words; otherwise consecutive butterflies.
-    constexpr bool LINE_FAST = (ES > 1 && LS == 1);
+    // Consecutive threads take consecutive lines.  With unit or odd line strides (every
+    // layout here: rows of M+1 complex values with M even) a wavefront then hits each LDS
+    // bank the minimum number of times; consecutive butterflies of one line would collide
+    // across rows.
+    constexpr bool LINE_FAST = (LS == 1 || LS % 2 == 1) && NL > 1;
 #pragma unroll
     for (int l0 = 0; l0 < NL; l0 += LPR) {
         const int nlr = (NL - l0) < LPR ? (NL - l0) : LPR;
@@ -123,7 +125,8 @@ __device__ __forceinline__ void csplit_r2c(cpx<T>* S, const cpx<T>* twN) {
     for (int i0 = 0; i0 < NROWS * HALF; i0 += NT) {
         const int i = i0 + threadIdx.x;
         if (i < NROWS * HALF) {
-            const int y = i / HALF, k = i % HALF;
+            // consecutive threads on consecutive rows (odd row stride: no bank conflicts)
+            const int y = i % NROWS, k = i / NROWS;
             cpx<T>* row = S + y * LD;
             const int km = (M - k) % M;
             const cpx<T> zk = row[k], zm = row[km];
@@ -145,7 +148,7 @@ __device__ __forceinline__ void csplit_c2r(cpx<T>* S, const cpx<T>* twN) {
     for (int i0 = 0; i0 < NROWS * HALF; i0 += NT) {
         const int i = i0 + threadIdx.x;
         if (i < NROWS * HALF) {
-            const int y = i / HALF, k = i % HALF;
+            const int y = i % NROWS, k = i / NROWS;
             cpx<T>* row = S + y * LD;
             cpx<T> xa = row[k], xb = row[M - k];
             if (k == 0) {
