@@ -234,6 +234,11 @@ int pf_system_autotune(pf_system* sys, int reps, int* sched, double* best_ms);
 int pf_batch_autotune(pf_batch* b, int reps, int* sched, int cap, int* count);
 /* Forget every tuned schedule (built-in defaults again). */
 int pf_schedule_reset(void);
+/* Set the schedule of one grid size and precision (elem_bytes 4 or 8) by hand:
+   sched = {slab, pencil, planar, split}, -1 for the default of an entry.  Pencil 3 is the
+   two-threads-per-column form (FP32, nz = 64).  Systems already built pick it up at their
+   next launch; batches rebuild on their next membership or cell change. */
+int pf_schedule_set(int elem_bytes, int nx, int ny, int nz, const int* sched);
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask /* nsys flags or NULL: all */,
                   pf_scft_results* results);

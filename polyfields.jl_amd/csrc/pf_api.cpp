@@ -537,6 +537,17 @@ int pf_system_autotune(pf_system* sys, int reps, int* sched, double* best_ms) {
     });
 }
 
+int pf_schedule_set(int elem_bytes, int nx, int ny, int nz, const int* sched) {
+    return guard([&] {
+        need(sched && (elem_bytes == 4 || elem_bytes == 8), "schedule");
+        pf::Schedule& s = pf::schedule(elem_bytes, nx, ny, nz);
+        s.slab = sched[0];
+        s.pencil = sched[1];
+        s.planar = sched[2];
+        s.split = sched[3];
+    });
+}
+
 int pf_schedule_reset(void) {
     return guard([&] { pf::schedule_reset(); });
 }

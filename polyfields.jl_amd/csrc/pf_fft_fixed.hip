@@ -507,6 +507,80 @@ __global__ void __launch_bounds__(NT) k_pencil_reg(const Lane<T>* __restrict__ l
     for (int z = 0; z < NZ; ++z) kb[(size_t)z * ncol] = v[z];
 }
 
+// z-pencil pass with two threads per column (FP32, NZ = 64): lane pair (2c, 2c+1) holds
+// the even and odd z samples of column c.  Each lane transforms its 32 samples in
+// registers; one radix-2 stage across the pair (adjacent-lane swap) completes the 64-point
+// transform, the LD multiply acts on the 32 wave numbers each lane then holds, and the
+// inverse runs the same steps backwards.  Half the registers of the one-thread form, so
+// two waves fit per SIMD.
+__device__ __forceinline__ float swap_pair(float x) {
+    // quad_perm [1, 0, 3, 2]: exchange with the adjacent lane
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_pencil_pair(const Lane<float>* __restrict__ lanes, int ny,
+                                                    int per, int t) {
+    constexpr int NZ = 64, H = 32;
+    const int lane = blockIdx.x / per;
+    const int c = (blockIdx.x - lane * per) * (NT / 2) + (threadIdx.x >> 1);
+    const int half = threadIdx.x & 1;
+    const Lane<float>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    const int nxh = (L->nx >> 1) + 1, ncol = nxh * ny;
+    const bool valid = c < ncol;
+    int s_in;
+    const int b = step_block(L, t, &s_in);
+    const auto kb = gp(L->kbuf + (valid ? c : 0));
+    cpx<float> v[H], u[H];
+#pragma unroll
+    for (int m = 0; m < H; ++m) {
+        const cpx<float> x = kb[(size_t)(2 * m + half) * ncol];
+        v[m] = valid ? x : cpx<float>{0.f, 0.f};
+    }
+    // even (half 0) / odd (half 1) 32-point transforms, then X[k] = E + w^k O,
+    // X[k+32] = E - w^k O
+    fft_reg<float, H, RegSplit<H>::R1, RegSplit<H>::R2, false>(v);
+#pragma unroll
+    for (int k = 0; k < H; ++k) u[k] = {swap_pair(v[k].x), swap_pair(v[k].y)};
+    const float sg = half ? -1.f : 1.f;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        const cpx<float> e = half ? u[k] : v[k], o = half ? v[k] : u[k];
+        const cpx<float> wo = cmul(twr<float, NZ, false>(k), o);
+        v[k] = {e.x + sg * wo.x, e.y + sg * wo.y};
+    }
+    const double mc = -L->ldc[b];
+    const double* G = L->G;
+    const int ky = c / nxh, kx = c - ky * nxh;
+    const double jx = (double)kx, jy = (double)fold(ky, ny);
+    const double a0 = G[0] * jx + G[1] * jy, a1 = G[3] * jx + G[4] * jy,
+                 a2 = G[6] * jx + G[7] * jy;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        const int kz = k + H * half;
+        const double jz = (double)(2 * kz <= NZ ? kz : NZ - kz);
+        const double k0 = a0 + G[2] * jz, k1 = a1 + G[5] * jz, k2 = a2 + G[8] * jz;
+        const float f = exp_t<float>(mc * (k0 * k0 + k1 * k1 + k2 * k2));
+        v[k].x *= f;
+        v[k].y *= f;
+    }
+    // inverse: even samples from X[k] + X[k+32], odd from (X[k] - X[k+32]) w^-k
+#pragma unroll
+    for (int k = 0; k < H; ++k) u[k] = {swap_pair(v[k].x), swap_pair(v[k].y)};
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        const cpx<float> lo = half ? u[k] : v[k], hi = half ? v[k] : u[k];
+        const cpx<float> d = half ? csub(lo, hi) : cadd(lo, hi);
+        v[k] = half ? cmul(d, twr<float, NZ, true>(k)) : d;
+    }
+    fft_reg<float, H, RegSplit<H>::R1, RegSplit<H>::R2, true>(v);
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < H; ++m) kb[(size_t)(2 * m + half) * ncol] = v[m];
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Schedules and dispatch: the dispatchers return false when the schedule asks for the
 // generic kernels or the size has no specialisation.
@@ -584,6 +658,15 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
                            dim3(256), 0, st, d, ny, t);                                     \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (pk == 3 && nz == 64) {
+            const int ncol = (nx / 2 + 1) * ny, per = (ncol + 31) / 32;
+            hipLaunchKernelGGL((k_pencil_pair<64>), dim3(nl * per), dim3(64), 0, st,
+                               reinterpret_cast<const Lane<float>*>(d), ny, per, t);
+            PF_HIP(hipGetLastError());
+            return true;
+        }
     }
     if (pk != 1) {
         // one thread per column, the whole z line in registers

@@ -160,7 +160,7 @@ void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, i
 // keeps the built-in default; pf_autotune times the candidates and writes the fastest.
 struct Schedule {
     int slab = -1;     // 0 generic, 1 compile-time kernel with 256 threads, 2 with 512
-    int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil
+    int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil
     int planar = -1;   // 0 generic, 1 compile-time planar kernel
     int split = -1;    // streams per 3-D lane group in a batch (1 or 2)
 };

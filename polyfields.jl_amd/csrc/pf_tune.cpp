@@ -29,11 +29,14 @@ static double time_ms(hipStream_t st, const std::function<void()>& run, int reps
     return ms / reps;
 }
 
-static std::vector<Schedule> candidates(int nz, bool batch) {
+static std::vector<Schedule> candidates(int elem, int nz, bool batch) {
     std::vector<Schedule> out;
     if (nz > 1) {
+        // the pair pencil exists for FP32 z lines of 64
+        const std::vector<int> pencils =
+            (elem == 4 && nz == 64) ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 1, 2};
         for (int sl : {0, 1, 2})
-            for (int pe : {0, 1, 2})
+            for (int pe : pencils)
                 for (int sp : batch ? std::vector<int>{1, 2} : std::vector<int>{-1}) {
                     Schedule c;
                     c.slab = sl;
@@ -58,7 +61,7 @@ static Schedule tune(int elem, const int* dims, hipStream_t st, bool batch,
     Schedule& sc = schedule(elem, dims[0], dims[1], dims[2]);
     Schedule best = sc;
     double tbest = std::numeric_limits<double>::infinity();
-    for (const Schedule& c : candidates(dims[2], batch)) {
+    for (const Schedule& c : candidates(elem, dims[2], batch)) {
         sc = c;
         changed();
         const double t = time_ms(st, run, reps);

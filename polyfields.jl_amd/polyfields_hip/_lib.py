@@ -88,6 +88,7 @@ SIGNATURES = {
     "pf_system_autotune": (I, [P, I, IP, DP]),
     "pf_batch_autotune": (I, [P, I, IP, I, IP]),
     "pf_schedule_reset": (I, []),
+    "pf_schedule_set": (I, [I, I, I, I, IP]),
     "pf_batch_timing_lanes": (I, [P, I, DP]),
     "pf_batch_timing_read": (I, [P, I, DP, IP]),
     "pf_system_timing": (I, [P, I]),

@@ -699,3 +699,10 @@ class Batch:
 def schedule_reset():
     """Forget every tuned kernel schedule (built-in defaults again)."""
     call("pf_schedule_reset")
+
+
+def schedule_set(precision, dims, slab=-1, pencil=-1, planar=-1, split=-1):
+    """Set the kernel schedule of one grid size by hand (-1: built-in default)."""
+    v = np.array([slab, pencil, planar, split], dtype=np.int32)
+    call("pf_schedule_set", 4 if precision == "fp32" else 8, int(dims[0]), int(dims[1]),
+         int(dims[2]), iptr(v))
