@@ -183,15 +183,22 @@ def batched_scft(P, count, plain_sample):
 # workload (64^3 FP32, 32 lanes per launch), scaled by the counters' reading for a plain
 # float4 copy of 68 MiB (FETCH_SIZE reads half the bytes copied, WRITE_SIZE all of them):
 # slab 2 * 31197.2 + 66228.7 KiB, pencil 2 * 17123.8 + 33792.0 KiB per 32-lane launch.
-TRAFFIC_PER_LANE = {"xy-slab pass": (2 * 31197.2 + 66228.7) * 1024 / 32,
-                    "z-pencil pass": (2 * 17123.8 + 33792.0) * 1024 / 32}
+# HBM bytes per lane and launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
+# 32-lane dispatch, FETCH x 2.0 and WRITE x 1.0 from a plain copy kernel of known size),
+# per xy-slab form: 3 the shuffle-exchange kernel (default), 2 the 512-thread LDS kernel
+TRAFFIC_PER_LANE = {("xy-slab pass", 3): (2 * 31202.0 + 66292.1) * 1024 / 32,
+                    ("xy-slab pass", 2): (2 * 31194.1 + 66228.7) * 1024 / 32,
+                    ("z-pencil pass", None): (2 * 17091.6 + 33792.0) * 1024 / 32}
 
 
-def traffic_per_launch(kernel, args, slab_lanes, pen_lanes):
+def traffic_per_launch(kernel, args, slab_lanes, pen_lanes, slab_form):
     if args.precision != "fp32" or GRID != 64:
         return None
-    lanes = slab_lanes if kernel == "xy-slab pass" else pen_lanes
-    return TRAFFIC_PER_LANE[kernel] * lanes
+    if kernel == "z-pencil pass":
+        return TRAFFIC_PER_LANE[(kernel, None)] * pen_lanes
+    form = 3 if slab_form < 0 else slab_form
+    per = TRAFFIC_PER_LANE.get((kernel, form))
+    return per * slab_lanes if per is not None else None
 
 
 def single_instance(P, args, nsteps_mde):
@@ -277,6 +284,8 @@ def main():
         ("z-pencil pass", pen_bytes, pen_avg)
     achieved = dom[1] / dom[2] if dom[2] > 0 else 0.0
     b_step = s * (3 * n + 4 * nk)
+    slab_form = next((c["slab"] for c in schedule if (c["nx"], c["ny"], c["nz"]) ==
+                      (GRID, GRID, GRID)), -1)
     line = {
         "metric": "64^3 MDE steps/s (FP32 propagator) + % HBM roofline",
         "value": value,
@@ -298,7 +307,8 @@ def main():
                    "precision": args.precision},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": traffic_per_launch(dom[0], args, slab_lanes, pen_lanes),
+                     "traffic": traffic_per_launch(dom[0], args, slab_lanes, pen_lanes,
+                                                   slab_form),
                      "bytes_per_launch": dom[1], "avg_launch_us": dom[2] * 1e6},
         "step_roofline_frac": value * b_step / HBM_PEAK,
         "kernel_avg_us": {"slab": slab_avg * 1e6, "pencil": pen_avg * 1e6},

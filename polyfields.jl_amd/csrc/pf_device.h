@@ -60,8 +60,12 @@ template <typename T>
 __device__ __forceinline__ T exp_t(double a);
 template <>
 __device__ __forceinline__ double exp_t<double>(double a) { return exp(a); }
+// FP32 sweeps: the hardware 2^x (v_exp_f32, about 1 ulp) of a log2(e), rounded once to
+// float, as expf would round its argument
 template <>
-__device__ __forceinline__ float exp_t<float>(double a) { return expf((float)a); }
+__device__ __forceinline__ float exp_t<float>(double a) {
+    return __builtin_amdgcn_exp2f((float)(a * 1.4426950408889634074));
+}
 
 // Small DFTs of length R in registers.  Forward uses exp(-2 pi i/R), inverse exp(+...).
 template <typename T, int R, bool INV>

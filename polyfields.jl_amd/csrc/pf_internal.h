@@ -159,7 +159,8 @@ void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, i
 // Kernel schedule of one grid size and precision, process-wide (pf_fft_fixed.hip).  -1
 // keeps the built-in default; pf_autotune times the candidates and writes the fastest.
 struct Schedule {
-    int slab = -1;     // 0 generic, 1 compile-time kernel with 256 threads, 2 with 512
+    int slab = -1;     // 0 generic, 1 compile-time kernel with 256 threads, 2 with 512,
+                       // 3 shuffle-exchange (FP32 64 x 64 planes)
     int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil
     int planar = -1;   // 0 generic, 1 compile-time planar kernel
     int split = -1;    // streams per 3-D lane group in a batch (1 or 2)

@@ -1,9 +1,10 @@
 // Autotuner: times the candidate kernel schedules of a grid size on the caller's own
 // workload (one density! per repetition) and keeps the fastest in the process-wide
 // schedule table that the launchers consult.  Candidates: for 3-D grids the xy-slab form
-// (generic, compile-time with 256 or 512 threads) x the z-pencil form (generic,
-// compile-time LDS, register-resident), and in batches the number of streams per lane
-// group; for 1-D/2-D grids the generic or compile-time planar kernel.
+// (generic, compile-time with 256 or 512 threads, shuffle-exchange) x the z-pencil form
+// (generic, compile-time LDS, register-resident, two threads per column), and in batches
+// the number of streams per lane group; for 1-D/2-D grids the generic or compile-time
+// planar kernel.
 #include <algorithm>
 #include <functional>
 #include <limits>
@@ -29,13 +30,18 @@ static double time_ms(hipStream_t st, const std::function<void()>& run, int reps
     return ms / reps;
 }
 
-static std::vector<Schedule> candidates(int elem, int nz, bool batch) {
+static std::vector<Schedule> candidates(int elem, const int* dims, bool batch) {
     std::vector<Schedule> out;
+    const int nz = dims[2];
     if (nz > 1) {
-        // the pair pencil exists for FP32 z lines of 64
+        // the pair pencil exists for FP32 z lines of 64, the shuffle-exchange slab for FP32
+        // 64 x 64 planes
+        const bool f64p = elem == 4 && dims[0] == 64 && dims[1] == 64;
         const std::vector<int> pencils =
             (elem == 4 && nz == 64) ? std::vector<int>{0, 1, 2, 3} : std::vector<int>{0, 1, 2};
-        for (int sl : {0, 1, 2})
+        const std::vector<int> slabs = f64p ? std::vector<int>{0, 1, 2, 3}
+                                            : std::vector<int>{0, 1, 2};
+        for (int sl : slabs)
             for (int pe : pencils)
                 for (int sp : batch ? std::vector<int>{1, 2} : std::vector<int>{-1}) {
                     Schedule c;
@@ -55,24 +61,35 @@ static std::vector<Schedule> candidates(int elem, int nz, bool batch) {
 }
 
 // Tune the schedule of (elem, dims) with `run` timing one density!; returns the best.
+// Every candidate is timed once; the four fastest are then timed again in interleaved
+// rounds and ranked by their best time, so that a few percent of run-to-run noise does not
+// decide between close candidates.
 static Schedule tune(int elem, const int* dims, hipStream_t st, bool batch,
                      const std::function<void()>& run, const std::function<void()>& changed,
                      int reps, double* best_ms) {
     Schedule& sc = schedule(elem, dims[0], dims[1], dims[2]);
-    Schedule best = sc;
-    double tbest = std::numeric_limits<double>::infinity();
-    for (const Schedule& c : candidates(elem, dims[2], batch)) {
-        sc = c;
+    const std::vector<Schedule> cand = candidates(elem, dims, batch);
+    std::vector<double> t(cand.size(), std::numeric_limits<double>::infinity());
+    auto measure = [&](size_t i) {
+        sc = cand[i];
         changed();
-        const double t = time_ms(st, run, reps);
-        if (t < tbest) {
-            tbest = t;
-            best = c;
-        }
-    }
+        t[i] = std::min(t[i], time_ms(st, run, reps));
+    };
+    for (size_t i = 0; i < cand.size(); ++i) measure(i);
+    std::vector<size_t> order(cand.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return t[x] < t[y]; });
+    order.resize(std::min<size_t>(order.size(), 4));
+    if (order.size() > 1)
+        for (int round = 0; round < 2; ++round)
+            for (size_t i : order) measure(i);
+    size_t bi = order.empty() ? 0 : order[0];
+    for (size_t i : order)
+        if (t[i] < t[bi]) bi = i;
+    const Schedule best = cand.empty() ? sc : cand[bi];
     sc = best;
     changed();
-    if (best_ms) *best_ms = tbest;
+    if (best_ms) *best_ms = cand.empty() ? 0.0 : t[bi];
     return best;
 }
 
@@ -90,8 +107,8 @@ void autotune_batch(Batch& b, int reps, std::vector<std::pair<std::array<int, 4>
         std::array<int, 3> d = {s->dims[0], s->dims[1], s->dims[2]};
         if (std::find(sizes.begin(), sizes.end(), d) == sizes.end()) sizes.push_back(d);
     }
-    const bool graph = b.use_graph;
-    b.use_graph = false;
+    // timed as the batch runs: captured and replayed when graphs are on (the first run of
+    // each candidate captures)
     auto changed = [&] {
         PF_HIP(hipStreamSynchronize(b.stream));
         b.drop_graphs();
@@ -103,7 +120,6 @@ void autotune_batch(Batch& b, int reps, std::vector<std::pair<std::array<int, 4>
                                 reps, &ms);
         out.push_back({{elem, d[0], d[1], d[2]}, s});
     }
-    b.use_graph = graph;
     changed();
 }
 

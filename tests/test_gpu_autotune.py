@@ -38,7 +38,8 @@ def test_system_autotune(dims, dim, crystal, precision, tol):
     sched, ms = s.autotune(reps=2)
     assert ms > 0
     if dims[2] > 1:
-        assert sched["slab"] in (0, 1, 2)
+        assert sched["slab"] in ((0, 1, 2, 3) if precision == "fp32" and dims[:2] == (64, 64)
+                                 else (0, 1, 2))
         assert sched["pencil"] in ((0, 1, 2, 3) if precision == "fp32" and dims[2] == 64
                                    else (0, 1, 2))
     else:
@@ -75,8 +76,9 @@ def test_batch_autotune_then_scft():
 
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("fp64", 1e-12)])
 def test_every_schedule_matches_default(precision, tol):
-    """Every slab x pencil candidate of a 64^3 grid (generic, compile-time, register and
-    pair pencils) gives the default schedule's densities to the transform rounding."""
+    """Every slab x pencil candidate of a 64^3 grid (generic, compile-time and
+    shuffle-exchange slabs; generic, LDS, register and pair pencils) gives the default
+    schedule's densities to the transform rounding."""
     dims = (64, 64, 64)
     s, _, _ = diblock_system(P, dims, 3, "cubic", (8.0,), N=100, f=0.35, chi=0.2, Ns=12,
                              precision=precision)
@@ -84,7 +86,8 @@ def test_every_schedule_matches_default(precision, tol):
     s.density_update()
     ref = {m: s.density[m] for m in s.mids}
     pencils = (0, 1, 2, 3) if precision == "fp32" else (0, 1, 2)
-    for slab in (0, 1, 2):
+    slabs = (0, 1, 2, 3) if precision == "fp32" else (0, 1, 2)
+    for slab in slabs:
         for pencil in pencils:
             P.schedule_set(precision, dims, slab=slab, pencil=pencil)
             seeded(s, 9)
