@@ -131,7 +131,7 @@ int pf_system_synchronize(pf_system* sys);
    every sum are float64 in both modes, so the state carries over; the q stacks are
    reallocated.  Used to polish an FP32-converged state in FP64. */
 int pf_system_set_precision(pf_system* sys, int precision);
-/* Propagator scheme: 0 RK2 (default), 1 RQM4 (mde.jl:83-104; 1-D and 2-D grids).
+/* Propagator scheme: 0 RK2 (default), 1 RQM4 (mde.jl:83-104; 1-D, 2-D and 3-D grids).
    Replaces the `mde` keyword of FieldSystem (system/system.jl:51-56). */
 int pf_system_set_mde(pf_system* sys, int method);
 

@@ -164,8 +164,6 @@ class Engine : public EngineBase {
         tile = (nxh + ntiles - 1) / ntiles;
         lds_plane = (int)(sizeof(cpx<T>) * ny * nxh);
         lds_pencil = (int)(sizeof(cpx<T>) * nz * tile);
-        if (s.mde == 1 && !planar)
-            throw Error(-2, "the RQM4 propagator is provided for 1-D and 2-D grids only");
         if (s.mde == 1 && 2 * lds_plane > 160 * 1024)
             throw Error(-2, "RQM4 needs two xy planes in LDS; this plane is too large");
         if (lds_plane > 160 * 1024)
@@ -225,6 +223,13 @@ class Engine : public EngineBase {
                 if (!planar) {
                     kbufs.push_back((cpx<T>*)dev_alloc(sizeof(cpx<T>) * nk));
                     L.kbuf = kbufs.back();
+                    if (s.mde == 1) {
+                        // RQM4: the half steps' spectrum and the full step's result
+                        kbufs.push_back((cpx<T>*)dev_alloc(sizeof(cpx<T>) * nk));
+                        L.kb2 = kbufs.back();
+                        kbufs.push_back((cpx<T>*)dev_alloc(sizeof(T) * n));
+                        L.rq = reinterpret_cast<T*>(kbufs.back());
+                    }
                 }
                 L.twx = twx;
                 L.twy = twy;
@@ -501,6 +506,18 @@ class Engine : public EngineBase {
                     else if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
                         launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st);
                 });
+            } else if (s.mde == 1) {
+                // RQM4: per contour step two spectra through the pencils and three slab
+                // launches (the next step's forward transforms ride on RQ_END)
+                const int nz = s.dims[2], ny = s.dims[1];
+                launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_START, 0, 2 * lds_plane, st);
+                for (int t = 0; t < maxNs; ++t) {
+                    launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 0);
+                    launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
+                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_MID, t, 2 * lds_plane, st);
+                    launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
+                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_END, t, 2 * lds_plane, st);
+                }
             } else {
                 const int nz = s.dims[2], ny = s.dims[1];
                 timed(0, [&] {

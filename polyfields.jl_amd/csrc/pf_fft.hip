@@ -194,11 +194,120 @@ __global__ void __launch_bounds__(256) k_slab(const Lane<T>* __restrict__ lanes,
     }
 }
 
+// 3-D lanes, RQM4 steps (src/system/mde.jl:83-104): the full step
+// q1 = LW1 F^-1 LD1 F LW1 q and two half steps, mid = LW1 F^-1 LD2 F LW2 q and
+// q2 = LW2 F^-1 LD2 F mid, give q_out = (4 q2 - q1) / 3.  The full step's spectrum is
+// kbuf, the half steps' kb2, and q1 waits in rq between launches.  Per contour step:
+//   RQ_START (first step only): kbuf <- F LW1 q_s, kb2 <- F LW2 q_s
+//   pencils: LD1 on kbuf, LD2 on kb2
+//   RQ_MID:  rq <- LW1 F^-1 kbuf;  kb2 <- F (LW1 F^-1 kb2)
+//   pencil:  LD2 on kb2
+//   RQ_END:  q_out <- (4 LW2 F^-1 kb2 - rq) / 3, then for the next step
+//            kbuf <- F LW1' q_out, kb2 <- F LW2' q_out
+// Two xy planes of LDS (S, S2).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_slab_rqm4(const Lane<T>* __restrict__ lanes, int nz,
+                                                   int t) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int lane = blockIdx.x / nz;
+    const int z = blockIdx.x - lane * nz;
+    const Lane<T>* L = lanes + lane;
+    if (t >= L->nsteps) return;
+    const int nx = L->nx, ny = L->ny, M = nx >> 1, ld = M + 1;
+    const int plane = nx * ny, n = plane * nz, np = plane >> 1;
+    const int nxh = M + 1, kplane = nxh * ny;
+    cpx<T>* S = reinterpret_cast<cpx<T>*>(smem_raw);
+    cpx<T>* S2 = S + ny * ld;
+    cpx<T>* kb = L->kbuf + (size_t)z * kplane;
+    cpx<T>* kb2 = L->kb2 + (size_t)z * kplane;
+    cpx<T>* rq = reinterpret_cast<cpx<T>*>(L->rq + (size_t)z * plane);
+    const T scale = T(2.0 / (double)n);
+    int s_in;
+    const int b = step_block(L, t, &s_in);
+    const cpx<T>* lw1 = reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * plane);
+    const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw2[b] + (size_t)z * plane);
+    auto spec_in = [&](cpx<T>* dst, const cpx<T>* src) {
+        for (int i = threadIdx.x; i < kplane; i += blockDim.x) {
+            const int y = i / nxh;
+            dst[y * ld + (i - y * nxh)] = src[i];
+        }
+    };
+    auto spec_out = [&](cpx<T>* dst, const cpx<T>* src) {
+        for (int i = threadIdx.x; i < kplane; i += blockDim.x) {
+            const int y = i / nxh;
+            dst[i] = src[y * ld + (i - y * nxh)];
+        }
+    };
+    // packed real pairs i of the plane <-> LDS offset o
+    auto pairs = [&](auto&& f) {
+        for (int i = threadIdx.x; i < np; i += blockDim.x) {
+            const int y = i / M;
+            f(i, y * ld + (i - y * M));
+        }
+    };
+    auto both_forward = [&] {
+        __syncthreads();
+        plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        plane_forward(S2, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        spec_out(kb, S);
+        spec_out(kb2, S2);
+    };
+    if constexpr (MODE == RQ_START) {
+        const cpx<T>* q = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
+                                                          (size_t)z * plane);
+        pairs([&](int i, int o) {
+            const cpx<T> v = q[i], w = lw1[i], w2 = lw2[i];
+            S[o] = {v.x * w.x, v.y * w.y};
+            S2[o] = {v.x * w2.x, v.y * w2.y};
+        });
+        both_forward();
+    } else if constexpr (MODE == RQ_MID) {
+        spec_in(S, kb);
+        spec_in(S2, kb2);
+        __syncthreads();
+        plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        plane_inverse(S2, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        pairs([&](int i, int o) {
+            const cpx<T> w = lw1[i], v = S[o], m = S2[o];
+            rq[i] = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+            S2[o] = {(m.x * scale) * w.x, (m.y * scale) * w.y};
+        });
+        __syncthreads();
+        plane_forward(S2, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        spec_out(kb2, S2);
+    } else {
+        spec_in(S2, kb2);
+        __syncthreads();
+        plane_inverse(S2, M, ny, ld, L->px, L->py, L->twx, L->twy);
+        cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
+                                                 (size_t)z * plane);
+        pairs([&](int i, int o) {
+            const cpx<T> w = lw2[i], v = S2[o], q1 = rq[i];
+            const cpx<T> q2 = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+            const cpx<T> r = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+            qout[i] = r;
+            S[o] = r;
+        });
+        if (t + 1 >= L->nsteps) return;
+        int s2;
+        const int b2 = step_block(L, t + 1, &s2);
+        const cpx<T>* lw1n = reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * plane);
+        const cpx<T>* lw2n = reinterpret_cast<const cpx<T>*>(L->lw2[b2] + (size_t)z * plane);
+        __syncthreads();
+        pairs([&](int i, int o) {
+            const cpx<T> v = S[o], w = lw1n[i], w2 = lw2n[i];
+            S[o] = {v.x * w.x, v.y * w.y};
+            S2[o] = {v.x * w2.x, v.y * w2.y};
+        });
+        both_forward();
+    }
+}
+
 // 3-D lanes, z-pencil pass: forward z, * LD_b(t), inverse z, for `tile` kx columns of
-// one ky row.
+// one ky row.  half: the RQM4 half-step spectrum kb2, LD at half the contour step.
 template <typename T>
 __global__ void __launch_bounds__(256) k_pencil(const Lane<T>* __restrict__ lanes, int ny,
-                                                int ntiles, int tile, int t) {
+                                                int ntiles, int tile, int t, int half) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T>* P = reinterpret_cast<cpx<T>*>(smem_raw);
     const int per = ny * ntiles;
@@ -212,7 +321,7 @@ __global__ void __launch_bounds__(256) k_pencil(const Lane<T>* __restrict__ lane
     const int tw = min(tile, nxh - kx0);
     int s_in;
     const int b = step_block(L, t, &s_in);
-    cpx<T>* kb = L->kbuf + (size_t)ky * nxh + kx0;
+    cpx<T>* kb = (half ? L->kb2 : L->kbuf) + (size_t)ky * nxh + kx0;
     const size_t zstride = (size_t)ny * nxh;
     for (int i = threadIdx.x; i < nz * tw; i += blockDim.x) {
         const int z = i / tw, c = i - z * tw;
@@ -220,7 +329,7 @@ __global__ void __launch_bounds__(256) k_pencil(const Lane<T>* __restrict__ lane
     }
     __syncthreads();
     lds_fft<T, false>(P, L->pz, tw, tile, 1, L->twz, 1);
-    const double mc = -L->ldc[b];
+    const double mc = half ? -L->ldc[b] / 2.0 : -L->ldc[b];
     const double jy = (double)fold(ky, ny);
     for (int i = threadIdx.x; i < nz * tw; i += blockDim.x) {
         const int z = i / tw, c = i - z * tw;
@@ -387,10 +496,26 @@ void launch_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, in
 
 template <typename T>
 void launch_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
-                   int lds_bytes, int threads, hipStream_t st) {
+                   int lds_bytes, int threads, hipStream_t st, int half) {
     set_lds(k_pencil<T>, lds_bytes);
     hipLaunchKernelGGL(k_pencil<T>, dim3(nlanes * ny * ntiles), dim3(threads), lds_bytes, st,
-                       d_lanes, ny, ntiles, tile, t);
+                       d_lanes, ny, ntiles, tile, t, half);
+    PF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_slab_rqm4(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t,
+                      int lds_bytes, hipStream_t st) {
+    auto go = [&](auto kern) {
+        set_lds(kern, lds_bytes);
+        hipLaunchKernelGGL(kern, dim3(nlanes * nz), dim3(256), lds_bytes, st, d_lanes, nz, t);
+    };
+    if (mode == RQ_START)
+        go(k_slab_rqm4<T, RQ_START>);
+    else if (mode == RQ_MID)
+        go(k_slab_rqm4<T, RQ_MID>);
+    else
+        go(k_slab_rqm4<T, RQ_END>);
     PF_HIP(hipGetLastError());
 }
 
@@ -426,7 +551,8 @@ void launch_stress_pencil(const StressLane<T>* d_lanes, int nlanes, int ny, int 
     template void launch_planar_steps<T>(const Lane<T>*, int, int, int, int, int, hipStream_t); \
     template void launch_slab<T>(const Lane<T>*, int, int, int, int, int, int, hipStream_t);  \
     template void launch_pencil<T>(const Lane<T>*, int, int, int, int, int, int, int,         \
-                                   hipStream_t);                                            \
+                                   hipStream_t, int);                                       \
+    template void launch_slab_rqm4<T>(const Lane<T>*, int, int, int, int, int, hipStream_t);   \
     template void launch_stress_planar<T>(const StressLane<T>*, int, double*, int, int,       \
                                           hipStream_t);                                     \
     template void launch_stress_slab<T>(const StressLane<T>*, int, int, int, int, hipStream_t); \

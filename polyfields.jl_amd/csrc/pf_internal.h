@@ -61,6 +61,8 @@ struct Lane {
     FftPlan px, py, pz;      // plans for nx/2, ny, nz
     int rqm4;                // 1: RQM4 step (Richardson of one full and two half steps)
     const T* lw2[PF_MAXBLK]; // exp(-ds_b/4 v w) per block (RQM4 only)
+    T* rq;                   // 3-D RQM4: the full step's result q1 (n entries); the half
+                             // steps' spectrum is kb2
     // --- adjoint sweeps (free-energy gradient); unused by the density sweeps --------
     const T* fwd;            // forward stack of the chain this lane is the adjoint of
     const T* other;          // partner stack of the Simpson source (q+ for q, q for q+)
@@ -131,9 +133,15 @@ void launch_planar_steps(const Lane<T>* d_lanes, int nlanes, int t0, int t1, int
 template <typename T>
 void launch_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t, int lds_bytes,
                  int threads, hipStream_t st);
+// half = 1: the RQM4 half-step spectrum kb2 with LD at half the contour step
 template <typename T>
 void launch_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
-                   int lds_bytes, int threads, hipStream_t st);
+                   int lds_bytes, int threads, hipStream_t st, int half = 0);
+// 3-D RQM4 xy-slab steps (two planes of LDS): RQ_START, RQ_MID, RQ_END
+enum RqmMode { RQ_START = 0, RQ_MID = 1, RQ_END = 2 };
+template <typename T>
+void launch_slab_rqm4(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t,
+                      int lds_bytes, hipStream_t st);
 template <typename T>
 void launch_stress_planar(const StressLane<T>* d_lanes, int nlanes, double* partial,
                           int lds_bytes, int threads, hipStream_t st);

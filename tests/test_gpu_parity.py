@@ -348,7 +348,10 @@ def test_batch_3d_density_fp32_matches_single():
 
 
 @pytest.mark.parametrize("dims,dim,crystal,params", [((32, 1, 1), 1, "lamellar", (9.0,)),
-                                                     ((24, 12, 1), 2, "rectangular", (7.0, 4.0))])
+                                                     ((24, 12, 1), 2, "rectangular", (7.0, 4.0)),
+                                                     ((8, 8, 8), 3, "cubic", (6.0,)),
+                                                     ((12, 8, 6), 3, "orthorhombic",
+                                                      (7.0, 5.0, 4.0))])
 def test_rqm4_propagator_fp64(dims, dim, crystal, params):
     def build(mod, precision=None):
         extra = {} if precision is None else {"precision": precision}
@@ -362,6 +365,21 @@ def test_rqm4_propagator_fp64(dims, dim, crystal, params):
     for m in (1, 2):
         assert rel(sg.density[m], so.density[m]) < 1e-12
     assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
+
+
+def test_rqm4_3d_fp32_error_vs_fp64():
+    # the 3-D RQM4 sweep in FP32 against the same sweep in FP64
+    dims = (16, 16, 16)
+    f = rand_fields((16, 16, 16))
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s, _, c = diblock_system(P, dims, 3, "cubic", (7.0,), Ns=30, mde="RQM4", precision=prec)
+        P.fieldinit(s, fields=f)
+        s.density_update()
+        out[prec] = (c.Q, {m: s.density[m] for m in (1, 2)})
+    assert out["fp32"][0] == pytest.approx(out["fp64"][0], rel=1e-5)
+    for m in (1, 2):
+        assert rel(out["fp32"][1][m], out["fp64"][1][m]) < 1e-5
 
 
 def test_fieldinit_from_reference_file_and_roundtrip(tmp_path):
