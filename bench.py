@@ -215,6 +215,94 @@ def single_instance(P, args, nsteps_mde):
     return nsteps_mde * args.steps / (time.perf_counter() - t0)
 
 
+# ---------------------------------------------------------------------------------
+# Other configurations, timed on request (--workload): the 128^3 blend and the 64^3
+# free-energy gradient.  Each prints one JSON line of its own.
+def blend_system(P, precision):
+    """128^3 orthorhombic cell, ABC triblock (N = 200, 60/80/60, b = 1.0/1.2/0.9, Ns = 200)
+    plus A homopolymer (N = 100, phi = 0.15); chi N = 25 / 20 / 35 at N = 200; cell edges
+    uniform in [6, 10] Rg from a fixed seed."""
+    nref = 200
+    rg = math.sqrt(nref / 6.0)
+    params = tuple(float(x) * rg for x in np.random.default_rng(5).uniform(6.0, 10.0, 3))
+    cell = P.UnitCell(3, "orthorhombic", *params)
+    A, B, C = P.Monomer(1), P.Monomer(2), P.Monomer(3)
+    tri = P.Multiblock([A, B, C], [60, 80, 60], [1.0, 1.2, 0.9], 200)
+    hom = P.Homopolymer(A, 100, 1.0, 100)
+    itx = P.FloryInteraction()
+    itx.set_interaction(1, 2, 25.0 / nref)
+    itx.set_interaction(2, 3, 20.0 / nref)
+    itx.set_interaction(1, 3, 35.0 / nref)
+    s = P.FieldSystem((128, 128, 128), cell, monomers=[A, B, C], precision=precision)
+    s.add_species(tri, 0.85)
+    s.add_species(hom, 0.15)
+    s.add_interaction(itx)
+    rng = np.random.default_rng(11)
+    for m in s.mids:
+        s.fields[m] = 0.1 * rng.standard_normal(s.shape)
+    return s
+
+
+def time_calls(fn, sync, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def blend_workload(P, args):
+    n = 128 ** 3
+    nk = 65 * 128 * 128
+    per_density = 2 * 200 + 100          # q and q+ of the triblock, q of the homopolymer
+    out = {}
+    for prec in ("fp32", "fp64"):
+        s = blend_system(P, prec)
+        dt = time_calls(s.density_update, s.synchronize, args.steps, args.warmup)
+        s.timing(True)
+        s.density_update()
+        slab_ms, slab_n = s.timing_read(0)
+        pen_ms, pen_n = s.timing_read(1)
+        s.timing(False)
+        b = 4 if prec == "fp32" else 8
+        out[prec] = {"ms_per_density": dt * 1e3, "mde_steps_per_s": per_density / dt,
+                     "step_roofline_frac": per_density / dt * b * (3 * n + 4 * nk) / HBM_PEAK,
+                     "kernel_avg_us": {"slab": slab_ms / max(slab_n, 1) * 1e3,
+                                       "pencil": pen_ms / max(pen_n, 1) * 1e3}}
+        del s
+    return {"metric": "128^3 blend MDE steps/s (FP32 propagator)",
+            "value": out["fp32"]["mde_steps_per_s"], "unit": "MDE steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": out["fp32"]["ms_per_density"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded 0.1*N(0,1))",
+            "config": {"workload": "128^3 orthorhombic ABC triblock (N=200, Ns=200) + A "
+                                   "homopolymer (N=100, phi=0.15); one density! (500 MDE "
+                                   "steps) per step; lanes: triblock q, q+, homopolymer q"},
+            "fp32": out["fp32"], "fp64": out["fp64"]}
+
+
+def grad_workload(P, args):
+    """64^3 cubic diblock with fields 0.5 N(0,1): one FP64 evaluation of fhelm and its
+    gradient (density! and the reverse sweeps; the gradient copied to the host)."""
+    s, ch = build(P, "fp64")
+    rng = np.random.default_rng(args.seed)
+    for m in s.mids:
+        s.fields[m] = 0.5 * rng.standard_normal(s.shape)
+    dt_d = time_calls(s.density_update, s.synchronize, args.steps, args.warmup)
+    dt_g = time_calls(lambda: P.fhelm_grad(s), s.synchronize, args.steps, args.warmup)
+    return {"metric": "64^3 free-energy gradient evaluations/s (FP64)", "value": 1.0 / dt_g,
+            "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": dt_g * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded 0.5*N(0,1))",
+            "config": {"workload": "64^3 cubic AB diblock N=100 fA=0.35 chiN=20 Ns=102; "
+                                   "fhelm + d/d(w, cell, chi, f) per step"},
+            "density_ms": dt_d * 1e3, "gradient_over_density": dt_g / dt_d}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +317,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--sweep-instances", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--workload", default="density64", choices=["density64", "blend128", "grad64"],
+                    help="density64: the headline (64^3 batch); blend128 / grad64: the large "
+                         "blend and the free-energy gradient, one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT backend runs on a single MI355X "
@@ -236,6 +327,11 @@ def main():
         sys.exit(2)
 
     import polyfields_hip as P
+
+    if args.workload != "density64":
+        fn = blend_workload if args.workload == "blend128" else grad_workload
+        print(json.dumps(fn(P, args)))
+        return
 
     systems = []
     for k in range(args.instances):
