@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "pf_fft_core.h"
+#include "pf_fft_fixed.h"
 
 namespace pf {
 
@@ -255,6 +256,237 @@ __global__ void __launch_bounds__(256) k_adj_pencil(const Lane<T>* __restrict__ 
         for (int k = 0; k < NG; ++k) gp[k] += out[k];
     }
 }
+
+// ---------------------------------------------------------------------------------
+// Compile-time forms of the two reverse kernels for the production plane and line sizes:
+// the same arithmetic, order of operations and Simpson sources as k_adj_slab /
+// k_adj_pencil, with fixed radix plans, twiddles staged in LDS and unrolled loops.
+template <typename T, int MODE, int NX, int NY, int NT>
+__global__ void __launch_bounds__(NT) k_adj_slab_fx(const Lane<T>* __restrict__ lanes, int nz,
+                                                    int t, int nextB) {
+    constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
+    __shared__ cpx<T> S[KPLANE];
+    __shared__ cpx<T> TWX[NX];
+    __shared__ cpx<T> TWY[NY];
+    const int lane = blockIdx.x / nz;
+    const int z = blockIdx.x - lane * nz;
+    const Lane<T>* L = lanes + lane;
+    const int NS = L->nsteps;
+    if (t >= NS) return;
+    const size_t n = (size_t)PLANE * nz, off = (size_t)z * PLANE;
+    cpx<T>* kA = L->kbuf + (size_t)z * KPLANE;
+    cpx<T>* kB = L->kb2 + (size_t)z * KPLANE;
+    lds_copy_in<T, NX, NT>(TWX, L->twx);
+    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    int s_in;
+    int b = step_block(L, NS - 1 - t, &s_in);
+    auto to_spectrum = [&](cpx<T>* kb) {
+        __syncthreads();
+        cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+#pragma unroll
+        for (int i0 = 0; i0 < KPLANE; i0 += NT)
+            if (i0 + (int)threadIdx.x < KPLANE) kb[i0 + threadIdx.x] = S[i0 + threadIdx.x];
+    };
+    auto lw_times_fwd = [&](int sl, int blk) {   // S <- LW_blk * a_sl
+        const cpx<T>* ai = reinterpret_cast<const cpx<T>*>(L->fwd + (size_t)(sl - 1) * n + off);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[blk] + off);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> q = ai[i], w = lw[i];
+            S[o] = {q.x * w.x, q.y * w.y};
+        });
+    };
+    if constexpr (MODE == 2) {
+        lw_times_fwd(s_in, b);
+        to_spectrum(kB);
+        return;
+    }
+    if constexpr (MODE == 0) {
+        const int s_out = s_in + L->dir;
+        const cpx<T>* ao = reinterpret_cast<const cpx<T>*>(L->fwd + (size_t)(s_out - 1) * n + off);
+        const cpx<T>* ot = reinterpret_cast<const cpx<T>*>(other_slice(L, s_out, n) + off);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + off);
+        double* wb = L->wbar + (size_t)b * n + off;
+        const double wc = L->wco[b];
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const size_t r0 = 2 * (size_t)i;
+            const cpx<T> ov = ot[i], a = ao[i], w = lw[i];
+            const double a0 = L->qinit + src_coef(L, s_out, off + r0, n) * (double)ov.x;
+            const double a1 = L->qinit + src_coef(L, s_out, off + r0 + 1, n) * (double)ov.y;
+            wb[r0] += wc * a0 * (double)a.x;
+            wb[r0 + 1] += wc * a1 * (double)a.y;
+            S[o] = {(T)a0 * w.x, (T)a1 * w.y};
+        });
+    } else {
+        lds_copy_in<T, KPLANE, NT>(S, kA);
+        __syncthreads();
+        cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
+        const T scale = T(2.0 / (double)n);
+        const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b] + off);
+        const cpx<T>* ai = reinterpret_cast<const cpx<T>*>(L->fwd + (size_t)(s_in - 1) * n + off);
+        const cpx<T>* ot = reinterpret_cast<const cpx<T>*>(other_slice(L, s_in, n) + off);
+        double* wb = L->wbar + (size_t)b * n + off;
+        const double wc = L->wco[b];
+        const bool more = t + 1 < NS;
+        int s2 = s_in, b2 = b;
+        if (more) b2 = step_block(L, NS - 2 - t, &s2);
+        double* wb2 = L->wbar + (size_t)b2 * n + off;
+        const double wc2 = L->wco[b2];
+        const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw[b2] + off);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const size_t r0 = 2 * (size_t)i;
+            cpx<T>& v = S[o];
+            const cpx<T> w = lw[i], a = ai[i];
+            const double p0 = (double)((v.x * scale) * w.x);
+            const double p1 = (double)((v.y * scale) * w.y);
+            double g0 = wc * (double)a.x * p0, g1 = wc * (double)a.y * p1;
+            if (more) {
+                const cpx<T> ov = ot[i], w2 = lw2[i];
+                const double a0 = p0 + src_coef(L, s_in, off + r0, n) * (double)ov.x;
+                const double a1 = p1 + src_coef(L, s_in, off + r0 + 1, n) * (double)ov.y;
+                if (b2 == b) {
+                    g0 += wc2 * a0 * (double)a.x;
+                    g1 += wc2 * a1 * (double)a.y;
+                } else {
+                    wb2[r0] += wc2 * a0 * (double)a.x;
+                    wb2[r0 + 1] += wc2 * a1 * (double)a.y;
+                }
+                v = {(T)a0 * w2.x, (T)a1 * w2.y};
+            }
+            wb[r0] += g0;
+            wb[r0 + 1] += g1;
+        });
+        if (!more) return;
+        b = b2;
+        s_in = s2;
+    }
+    to_spectrum(kA);
+    if (!nextB) return;
+    __syncthreads();
+    lw_times_fwd(s_in, b);
+    to_spectrum(kB);
+}
+
+// Reverse z pass over KB ky rows: forward z of A and B, the k-space sums and LD on A,
+// inverse z of A.  Partial-sum row r = the workgroup's index within its lane.
+template <typename T, int NZ, int NXH, int KB, int NT>
+__global__ void __launch_bounds__(NT) k_adj_pencil_fx(const Lane<T>* __restrict__ lanes, int ny,
+                                                      int t) {
+    constexpr int C = KB * NXH;
+    __shared__ cpx<T> PA[NZ * C];
+    __shared__ cpx<T> PB[NZ * C];
+    __shared__ cpx<T> TWZ[NZ];
+    const int per = ny / KB;
+    const int lane = blockIdx.x / per;
+    const int r = blockIdx.x - lane * per;
+    const int ky0 = r * KB;
+    const Lane<T>* L = lanes + lane;
+    const int NS = L->nsteps;
+    if (t >= NS) return;
+    lds_copy_in<T, NZ, NT>(TWZ, L->twz);
+    int s_in;
+    const int b = step_block(L, NS - 1 - t, &s_in);
+    constexpr int M = NXH - 1;
+    cpx<T>* kA = L->kbuf + (size_t)ky0 * NXH;
+    const cpx<T>* kB = L->kb2 + (size_t)ky0 * NXH;
+    const size_t zstride = (size_t)ny * NXH;
+#pragma unroll
+    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NZ * C) {
+            const size_t g = (size_t)(i / C) * zstride + (i % C);
+            PA[i] = kA[g];
+            PB[i] = kB[g];
+        }
+    }
+    __syncthreads();
+    cfft<T, NZ, C, C, 1, NT, false, 1>(PA, TWZ);
+    cfft<T, NZ, C, C, 1, NT, false, 1>(PB, TWZ);
+    const double mc = -L->ldc[b];
+    const int npar = L->nparams;
+    double acc[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NZ * C) {
+            const int z = i / C, c = i % C;
+            const int ky = ky0 + c / NXH, kx = c % NXH;
+            const double jy = (double)fold(ky, ny), jz = (double)fold(z, NZ);
+            const double k2 = ksq_at(L->G, (double)kx, jy, jz);
+            const T f = exp_t<T>(mc * k2);
+            cpx<T>& A = PA[i];
+            const cpx<T> B = PB[i];
+            const double m = (kx == 0 || kx == M) ? 1.0 : 2.0;
+            const double g = m * ((double)A.x * (double)B.x + (double)A.y * (double)B.y) * (double)f;
+            for (int k = 0; k < npar; ++k)
+                acc[k] += g * dksq_at(L->dGG, npar, k, L->dim, (double)kx, jy, jz);
+            acc[PF_MAXPAR] += g * k2;
+            A.x *= f;
+            A.y *= f;
+        }
+    }
+    __syncthreads();
+    cfft<T, NZ, C, C, 1, NT, true, 1>(PA, TWZ);
+#pragma unroll
+    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < NZ * C) kA[(size_t)(i / C) * zstride + (i % C)] = PA[i];
+    }
+    double out[NG];
+    wg_sum<NG>(acc, out);
+    if (threadIdx.x == 0) {
+        double* gp = L->gpart + ((size_t)r * L->nblk + b) * NG;
+        for (int k = 0; k < NG; ++k) gp[k] += out[k];
+    }
+}
+
+template <typename T>
+bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
+                    hipStream_t st, int nextB) {
+    if (schedule(sizeof(T), nx, ny, nz).slab == 0) return false;
+#define PF_AS(X, Y)                                                                         \
+    if (nx == X && ny == Y) {                                                               \
+        if (mode == 0)                                                                      \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 0, X, Y, 256>), dim3(nl * nz), dim3(256), 0, \
+                               st, d, nz, t, nextB);                                        \
+        else if (mode == 1)                                                                 \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 1, X, Y, 256>), dim3(nl * nz), dim3(256), 0, \
+                               st, d, nz, t, nextB);                                        \
+        else                                                                                \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 2, X, Y, 256>), dim3(nl * nz), dim3(256), 0, \
+                               st, d, nz, t, 1);                                            \
+        PF_HIP(hipGetLastError());                                                          \
+        return true;                                                                        \
+    }
+    PF_AS(32, 32)
+    PF_AS(64, 64)
+#undef PF_AS
+    return false;
+}
+
+template <typename T>
+int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hipStream_t st) {
+    if (schedule(sizeof(T), nx, ny, nz).pencil == 0) return 0;
+#define PF_AP(X, Z, KB)                                                                     \
+    if (nx == X && nz == Z && ny % KB == 0) {                                               \
+        hipLaunchKernelGGL((k_adj_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(nl * (ny / KB)),  \
+                           dim3(256), 0, st, d, ny, t);                                     \
+        PF_HIP(hipGetLastError());                                                          \
+        return ny / KB;                                                                     \
+    }
+    PF_AP(32, 32, 2)
+    PF_AP(64, 64, 1)
+#undef PF_AP
+    return 0;
+}
+
+template bool fixed_adj_slab<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t,
+                                    int);
+template bool fixed_adj_slab<double>(const Lane<double>*, int, int, int, int, int, int,
+                                     hipStream_t, int);
+template int fixed_adj_pencil<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
+template int fixed_adj_pencil<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
 
 template <typename Kern>
 static void set_lds_adj(Kern k, int bytes) {

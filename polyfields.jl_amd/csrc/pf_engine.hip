@@ -637,11 +637,29 @@ class Engine : public EngineBase {
 
     // Reverse sweeps on checkpointed stacks: each adjoint lane walks the segments in its
     // own order (q-bar from the top, q+-bar from the bottom); before a segment, q and q+
+    // Reverse-sweep launches: the compile-time kernels when the size has them (and the
+    // schedule does not ask for the generic ones), else the generic ones.  apper records
+    // the pencil workgroups per lane, i.e. the rows of k-space partial sums.
+    int apper = 0;
+    void adj_slab(const Lane<T>* d, int nl, int mode, int t, int nextB = 1) {
+        const System& s = *S;
+        if (!fixed || !fixed_adj_slab<T>(d, nl, s.dims[0], s.dims[1], s.dims[2], mode, t,
+                                         s.stream, nextB))
+            launch_adj_slab<T>(d, nl, s.dims[2], mode, t, lds_plane, s.stream, nextB);
+    }
+    void adj_pencil(const Lane<T>* d, int nl, int t) {
+        const System& s = *S;
+        const int pp = fixed ? fixed_adj_pencil<T>(d, nl, s.dims[0], s.dims[1], s.dims[2], t,
+                                                   s.stream)
+                             : 0;
+        if (!pp) launch_adj_pencil<T>(d, nl, s.dims[1], ntiles, tile, t, 2 * lds_pencil, s.stream);
+        apper = pp ? pp : s.dims[1] * ntiles;
+    }
+
     // of that segment are recomputed from their checkpoints.  At a segment edge the
     // next step's B spectrum is made by a separate launch once its input slice exists.
     void adjoint_ckpt(System& s) {
         const size_t n = s.n;
-        const int ny = s.dims[1], nz = s.dims[2];
         hipStream_t st = s.stream;
         std::vector<Lane<T>> aseg;
         std::vector<int> aoff(alanes.size(), 0);
@@ -673,11 +691,10 @@ class Engine : public EngineBase {
                     const int t0 = which == 0 ? NS + 1 - hi : lo - 1;
                     const int t1 = which == 0 ? NS + 1 - lo : hi - 1;
                     const Lane<T>* dA = d_aseg + aoff[j] + m;
-                    launch_adj_slab<T>(dA, 1, nz, k == 0 ? 0 : 2, t0, lds_plane, st);
+                    adj_slab(dA, 1, k == 0 ? 0 : 2, t0);
                     for (int t = t0; t < t1; ++t) {
-                        launch_adj_pencil<T>(dA, 1, ny, ntiles, tile, t, 2 * lds_pencil, st);
-                        launch_adj_slab<T>(dA, 1, nz, 1, t, lds_plane, st,
-                                           (t + 1 < t1 || k == nseg - 1) ? 1 : 0);
+                        adj_pencil(dA, 1, t);
+                        adj_slab(dA, 1, 1, t, (t + 1 < t1 || k == nseg - 1) ? 1 : 0);
                     }
                 }
             }
@@ -710,8 +727,7 @@ class Engine : public EngineBase {
         const int nl = (int)lanes.size();
         const size_t n = s.n;
         const int np = (int)s.cell.params.size();
-        const int ny = s.dims[1], nz = s.dims[2];
-        const int per = ny * ntiles;
+        const int per = s.dims[1] * ntiles;
         constexpr int NG = PF_MAXPAR + 1;
         hipStream_t st = s.stream;
         io.dtheta.assign(np, 0.0);
@@ -765,11 +781,12 @@ class Engine : public EngineBase {
             launch_fill<double>(A.gpart, 0.0, (size_t)per * sp.nblk() * NG, st);
         }
         upload(d_alanes, alanes.data(), sizeof(Lane<T>) * nl);
+        apper = per;
         if (!ckK) {
-            launch_adj_slab<T>(d_alanes, nl, nz, 0, 0, lds_plane, st);
+            adj_slab(d_alanes, nl, 0, 0);
             for (int t = 0; t < maxNs; ++t) {
-                launch_adj_pencil<T>(d_alanes, nl, ny, ntiles, tile, t, 2 * lds_pencil, st);
-                launch_adj_slab<T>(d_alanes, nl, nz, 1, t, lds_plane, st);
+                adj_pencil(d_alanes, nl, t);
+                adj_slab(d_alanes, nl, 1, t);
             }
         } else {
             adjoint_ckpt(s);
@@ -809,13 +826,13 @@ class Engine : public EngineBase {
             const int i = lane_sp[j];
             const Species& sp = s.species[i];
             const int nb = sp.nblk();
-            std::vector<double> h((size_t)per * nb * NG);
+            std::vector<double> h((size_t)apper * nb * NG);
             PF_HIP(hipMemcpyAsync(h.data(), alanes[j].gpart, sizeof(double) * h.size(),
                                   hipMemcpyDeviceToHost, st));
             PF_HIP(hipStreamSynchronize(st));
             for (int k = 0; k < nb; ++k) {
                 double G[NG] = {0};
-                for (int r = 0; r < per; ++r)
+                for (int r = 0; r < apper; ++r)
                     for (int g = 0; g < NG; ++g) G[g] += h[((size_t)r * nb + k) * NG + g];
                 const double ldc = lanes[j].ldc[k];
                 for (int p = 0; p < np; ++p) io.dtheta[p] += -(ldc / (double)n) * G[p];

@@ -163,6 +163,14 @@ void launch_adj_slab(const Lane<T>* d_lanes, int nlanes, int nz, int mode, int t
 template <typename T>
 void launch_adj_pencil(const Lane<T>* d_lanes, int nlanes, int ny, int ntiles, int tile, int t,
                        int lds_bytes, hipStream_t st);
+// compile-time forms (pf_adjoint.hip): false / 0 when the size has none or the schedule
+// asks for the generic kernels; the pencil returns its workgroups per lane (partial-sum
+// rows)
+template <typename T>
+bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
+                    hipStream_t st, int nextB = 1);
+template <typename T>
+int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hipStream_t st);
 
 // Kernel schedule of one grid size and precision, process-wide (pf_fft_fixed.hip).  -1
 // keeps the built-in default; pf_autotune times the candidates and writes the fastest.

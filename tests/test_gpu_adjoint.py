@@ -246,6 +246,45 @@ def test_cfg4_fp32_gradient_error(cfg4_state):
 # --------------------------------------------------------------------------------------
 # the reference's analytic stress and SCF residual
 
+@pytest.mark.parametrize("case", ["tric32", "hex64x64", "blend32_ckpt"])
+def test_compile_time_reverse_kernels_match_generic(case):
+    # the reverse sweep through the compile-time kernels (32^2 / 64^2 planes, 32 / 64
+    # lines) against the generic kernels of the same system (schedule 0 / 0)
+    def make():
+        if case == "tric32":
+            s = diblock(P, (32, 32, 32), 3, "triclinic", (9.0, 8.5, 8.0, 1.4, 1.5, 1.7), Ns=16)
+        elif case == "hex64x64":
+            s = diblock(P, (64, 64, 1), 2, "hexagonal", (12.0,), Ns=16)
+        else:
+            cell = P.UnitCell(3, "orthorhombic", 9.0, 8.0, 7.0)
+            A, B = P.Monomer(1), P.Monomer(2)
+            tri = P.Multiblock([A, B, A], [12, 20, 8], [1.0, 1.1, 1.0], 20)
+            hom = P.Homopolymer(B, 30, 0.9, 16)
+            fl = P.FloryInteraction()
+            fl.set_interaction(1, 2, 0.3)
+            s = P.FieldSystem((32, 32, 32), cell, monomers=[A, B])
+            s.add_species(tri, 0.7)
+            s.add_species(hom, 0.3)
+            s.add_interaction(fl)
+            s.set_checkpoint(-1)
+        rng = np.random.default_rng(3)
+        P.fieldinit(s, fields={m: 0.4 * rng.standard_normal(s.shape) for m in s.mids})
+        return s
+    dims = {"tric32": (32, 32, 32), "hex64x64": (64, 64, 1), "blend32_ckpt": (32, 32, 32)}[case]
+    try:
+        P.schedule_set("fp64", dims, slab=0, pencil=0)
+        gg = P.fhelm_grad(make(), "textbook")
+    finally:
+        P.schedule_reset()
+    gf = P.fhelm_grad(make(), "textbook")
+    assert gf.fhelm == pytest.approx(gg.fhelm, rel=1e-13)
+    for m in gg.dw:
+        assert np.max(np.abs(gf.dw[m] - gg.dw[m])) <= 1e-11 * np.max(np.abs(gg.dw[m]))
+    np.testing.assert_allclose(gf.dcell, gg.dcell, rtol=1e-10, atol=1e-14)
+    for k in gg.dchi:
+        assert gf.dchi[k] == pytest.approx(gg.dchi[k], rel=1e-10, abs=1e-14)
+
+
 def test_dlogQ_dtheta_matches_reference_stress_at_second_order():
     """scfstress (the reference's formula, exact for this centrosymmetric 1-D field) is
     the continuum limit of d log Q / d theta from the reverse sweep."""
