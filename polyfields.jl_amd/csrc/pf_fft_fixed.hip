@@ -749,7 +749,7 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
         }
     }
     PF_SL1(64, 64, 512)
-    if constexpr (sizeof(T) == 4) { PF_SL(128, 128) }
+    PF_SL(128, 128)
 #undef PF_SL
 #undef PF_SL1
     return false;
@@ -798,7 +798,7 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
     }
     PF_PE(32, 32, 4)
     PF_PE(64, 64, 2)
-    if constexpr (sizeof(T) == 4) { PF_PE(128, 128, 1) }
+    PF_PE(128, 128, 1)
 #undef PF_PE
     return false;
 }
