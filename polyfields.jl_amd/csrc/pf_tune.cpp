@@ -4,7 +4,7 @@
 // (generic, compile-time with 256 or 512 threads, shuffle-exchange) x the z-pencil form
 // (generic, compile-time LDS, register-resident, two threads per column), and in batches
 // the number of streams per lane group; for 1-D/2-D grids the generic or compile-time
-// planar kernel.
+// planar kernel (2-D planes with 256, 512 or 1024 threads).
 #include <algorithm>
 #include <functional>
 #include <limits>
@@ -51,7 +51,7 @@ static std::vector<Schedule> candidates(int elem, const int* dims, bool batch) {
                     out.push_back(c);
                 }
     } else {
-        for (int pl : {0, 1}) {
+        for (int pl : {0, 1, 2, 3}) {
             Schedule c;
             c.planar = pl;
             out.push_back(c);

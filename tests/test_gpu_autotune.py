@@ -43,7 +43,7 @@ def test_system_autotune(dims, dim, crystal, precision, tol):
         assert sched["pencil"] in ((0, 1, 2, 3) if precision == "fp32" and dims[2] == 64
                                    else (0, 1, 2))
     else:
-        assert sched["planar"] in (0, 1)
+        assert sched["planar"] in (0, 1, 2, 3)
     seeded(s)
     s.density_update()
     for m in s.mids:
@@ -117,3 +117,22 @@ def test_fp64_128_compile_time_matches_generic():
     for m in out[0][1]:
         err = np.max(np.abs(out[1][1][m] - out[0][1][m])) / np.max(np.abs(out[0][1][m]))
         assert err < 1e-12, (m, err)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("fp64", 1e-12)])
+def test_every_planar_form_matches_default(precision, tol):
+    # the generic planar kernel and the compile-time one with 256, 512 and 1024 threads on
+    # a 96^2 hexagonal plane
+    dims = (96, 96, 1)
+    s, _, _ = diblock_system(P, dims, 2, "hexagonal", (9.0,), N=100, f=0.35, chi=0.2, Ns=24,
+                             precision=precision)
+    seeded(s, 5)
+    s.density_update()
+    ref = {m: s.density[m] for m in s.mids}
+    for planar in (0, 1, 2, 3):
+        P.schedule_set(precision, dims, planar=planar)
+        seeded(s, 5)
+        s.density_update()
+        for m in s.mids:
+            err = np.max(np.abs(s.density[m] - ref[m])) / np.max(np.abs(ref[m]))
+            assert err < tol, (planar, err)
