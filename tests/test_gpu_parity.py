@@ -367,6 +367,39 @@ def test_rqm4_propagator_fp64(dims, dim, crystal, params):
     assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
 
 
+def test_grand_ensemble_parity_fp64():
+    # diblock + homopolymer + point species at fixed chemical potentials
+    # (system.jl:248-259, 317-340 without the canonical mean subtraction)
+    def make(mod):
+        cell = mod.UnitCell(2, "rectangular", 6.0, 4.5)
+        a, b, c = mod.Monomer(1), mod.Monomer(2), mod.Monomer(3)
+        dib = mod.Diblock(a, b, 40, 0.5, 1.0, 1.0, 20)
+        hom = mod.Homopolymer(a, 20, 1.0, 10)
+        pt = mod.Point(c)
+        fl = mod.FloryInteraction()
+        fl.set_interaction(1, 2, 0.4)
+        fl.set_interaction(1, 3, 0.2)
+        s = mod.FieldSystem((16, 12, 1), cell, monomers=[a, b, c], ensemble="grand")
+        for sp, v in zip((dib, hom, pt), (0.1, -0.3, -1.2)):
+            s.add_species(sp, v)
+        s.add_interaction(fl)
+        return s
+    f = rand_fields((1, 12, 16), mids=(1, 2, 3))
+    so, sg = make(O), make(P)
+    O.fieldinit(so, fields=f)
+    P.fieldinit(sg, fields=f)
+    for isp in range(3):
+        q, phi, mu = sg.species_scalars(isp)
+        assert q == pytest.approx(so.species[isp].Q, rel=1e-12)
+        assert phi == pytest.approx(so.phi_species[isp], rel=1e-12)
+        assert mu == pytest.approx(so.mu_species[isp], rel=1e-14)
+    for m in (1, 2, 3):
+        assert rel(sg.density[m], so.density[m]) < 1e-12
+        assert rel(sg.residuals[m], so.residuals[m]) < 1e-10
+        assert rel(sg.fields[m], so.fields[m]) < 1e-14
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
+
+
 def test_rqm4_3d_fp32_error_vs_fp64():
     # the 3-D RQM4 sweep in FP32 against the same sweep in FP64
     dims = (16, 16, 16)

@@ -185,3 +185,45 @@ def test_rqm4_fourth_order_and_symmetric():
     e = [abs(Qs[i + 1] - Qs[i]) for i in range(3)]
     r = [e[0] / e[1], e[1] / e[2]]
     assert 10.0 < r[0] < r[1] < 17.0
+
+
+def _grand_blend(mod, ensemble, vals, dims=(16, 12, 1)):
+    cell = mod.UnitCell(2, "rectangular", 6.0, 4.5)
+    a, b, c = mod.Monomer(1), mod.Monomer(2), mod.Monomer(3)
+    dib = mod.Diblock(a, b, 40, 0.5, 1.0, 1.0, 20)
+    hom = mod.Homopolymer(a, 20, 1.0, 10)
+    pt = mod.Point(c)
+    fl = mod.FloryInteraction()
+    fl.set_interaction(1, 2, 0.4)
+    fl.set_interaction(1, 3, 0.2)
+    s = mod.FieldSystem(dims, cell, monomers=[a, b, c], ensemble=ensemble)
+    for sp, v in zip((dib, hom, pt), vals):
+        s.add_species(sp, v)
+    s.add_interaction(fl)
+    return s
+
+
+def test_grand_matches_canonical_at_matching_mu():
+    # muphi! (system.jl:248-259): the grand ensemble with mu_p = log(phi_p / Q_p) taken from
+    # a canonical evaluation at the same fields gives back phi_p and the same densities;
+    # residuals differ only by the canonical mean subtraction (system.jl:332-337)
+    rng = np.random.default_rng(21)
+    f = {m: 0.3 * rng.standard_normal((1, 12, 16)) for m in (1, 2, 3)}
+    for m in f:
+        f[m] -= f[m].mean()
+    can = _grand_blend(O, "canonical", (0.5, 0.3, 0.2))
+    O.fieldinit(can, fields={m: w.copy() for m, w in f.items()})
+    Q = [sp.Q for sp in can.species]
+    mu = [math.log(p / q) for p, q in zip(can.phi_species, Q)]
+    gr = _grand_blend(O, "grand", mu)
+    O.fieldinit(gr, fields={m: w.copy() for m, w in f.items()})
+    for isp in range(3):
+        assert gr.phi_species[isp] == pytest.approx(can.phi_species[isp], rel=1e-12)
+    for m in (1, 2, 3):
+        np.testing.assert_allclose(gr.density[m], can.density[m], rtol=1e-12, atol=1e-15)
+        rg = gr.residuals[m] - gr.residuals[m].mean()
+        np.testing.assert_allclose(rg, can.residuals[m], rtol=1e-10, atol=1e-13)
+    # point species, closed form: Q = <exp(-v w)>, phi = Q exp(mu)
+    wq = np.mean(np.exp(-gr.fields[3]))
+    assert gr.species[2].Q == pytest.approx(wq, rel=1e-13)
+    assert gr.phi_species[2] == pytest.approx(wq * math.exp(mu[2]), rel=1e-13)
