@@ -176,8 +176,9 @@ int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hi
 // keeps the built-in default; pf_autotune times the candidates and writes the fastest.
 struct Schedule {
     int slab = -1;     // 0 generic, 1 compile-time kernel with 256 threads, 2 with 512,
-                       // 3 shuffle-exchange (FP32 64 x 64 planes)
-    int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil
+                       // 3 shuffle-exchange (FP32 64 x 64 planes), 4 1024 threads (FP32 128^2)
+    int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil,
+                       // 4 / 5 LDS pencil with 512 / 1024 threads (FP32 128-point lines)
     int planar = -1;   // 0 generic, compile-time planar kernel: 1 (2-D: 256 threads),
                        // 2 (2-D: 512), 3 (2-D: 1024)
     int split = -1;    // streams per 3-D lane group in a batch (1 or 2)

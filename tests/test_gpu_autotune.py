@@ -136,3 +136,20 @@ def test_every_planar_form_matches_default(precision, tol):
         for m in s.mids:
             err = np.max(np.abs(s.density[m] - ref[m])) / np.max(np.abs(ref[m]))
             assert err < tol, (planar, err)
+
+
+def test_fp32_128_forms_match():
+    # the FP32 128^3 slab (256, 512, 1024 threads) and LDS pencil (256, 512, 1024 threads)
+    dims = (128, 128, 128)
+    s, _, _ = diblock_system(P, dims, 3, "cubic", (14.0,), N=100, f=0.35, chi=0.2, Ns=8,
+                             precision="fp32")
+    seeded(s, 6)
+    s.density_update()
+    ref = {m: s.density[m] for m in s.mids}
+    for slab, pencil in ((1, 1), (2, 4), (4, 5), (0, 0)):
+        P.schedule_set("fp32", dims, slab=slab, pencil=pencil)
+        seeded(s, 6)
+        s.density_update()
+        for m in s.mids:
+            err = np.max(np.abs(s.density[m] - ref[m])) / np.max(np.abs(ref[m]))
+            assert err < 2e-5, (slab, pencil, err)
