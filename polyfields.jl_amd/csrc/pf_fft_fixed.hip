@@ -803,7 +803,7 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
             return true;
         }
     }
-    if (pk != 1) {
+    if (pk != 1 && pk != 4 && pk != 5) {
         // one thread per column, the whole z line in registers
         const int ncol = (nx / 2 + 1) * ny;
 #define PF_PR(Z)                                                                            \
@@ -823,6 +823,9 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
 #undef PF_PR
     }
     PF_PE(32, 32, 4)
+    // 64-point lines: two ky rows per workgroup, or one (form 4, the FP64 default: twice
+    // the workgroups; measured on MI355X, one 64^3 system's density! 2.83 -> 2.42 ms)
+    if (pk == 4 || (pk < 0 && sizeof(T) == 8)) { PF_PE(64, 64, 1) }
     PF_PE(64, 64, 2)
     if constexpr (sizeof(T) == 4) {
         // 128-point lines: the LDS pencil with 512 or 1024 threads

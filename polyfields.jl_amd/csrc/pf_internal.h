@@ -178,7 +178,8 @@ struct Schedule {
     int slab = -1;     // 0 generic, 1 compile-time kernel with 256 threads, 2 with 512,
                        // 3 shuffle-exchange (FP32 64 x 64 planes), 4 1024 threads (FP32 128^2)
     int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil,
-                       // 4 / 5 LDS pencil with 512 / 1024 threads (FP32 128-point lines)
+                       // 4 LDS pencil with one ky row per workgroup (64-point lines) or
+                       // 512 threads (FP32 128-point lines), 5 with 1024 (FP32 128)
     int planar = -1;   // 0 generic, compile-time planar kernel: 1 (2-D: 256 threads),
                        // 2 (2-D: 512), 3 (2-D: 1024)
     int split = -1;    // streams per 3-D lane group in a batch (1 or 2)

@@ -42,6 +42,7 @@ static std::vector<Schedule> candidates(int elem, const int* dims, bool batch) {
         const bool p128 = f32 && dims[0] == 128 && dims[1] == 128;
         std::vector<int> pencils = {0, 1, 2};
         if (f32 && nz == 64) pencils.push_back(3);
+        if (nz == 64 && dims[0] == 64) pencils.push_back(4);
         if (f32 && nz == 128 && dims[0] == 128) {
             pencils.push_back(4);
             pencils.push_back(5);

@@ -40,8 +40,7 @@ def test_system_autotune(dims, dim, crystal, precision, tol):
     if dims[2] > 1:
         assert sched["slab"] in ((0, 1, 2, 3) if precision == "fp32" and dims[:2] == (64, 64)
                                  else (0, 1, 2))
-        assert sched["pencil"] in ((0, 1, 2, 3) if precision == "fp32" and dims[2] == 64
-                                   else (0, 1, 2))
+        assert sched["pencil"] in ((0, 1, 2, 3, 4) if dims[2] == 64 else (0, 1, 2))
     else:
         assert sched["planar"] in (0, 1, 2, 3)
     seeded(s)
@@ -85,7 +84,7 @@ def test_every_schedule_matches_default(precision, tol):
     seeded(s, 9)
     s.density_update()
     ref = {m: s.density[m] for m in s.mids}
-    pencils = (0, 1, 2, 3) if precision == "fp32" else (0, 1, 2)
+    pencils = (0, 1, 2, 3, 4) if precision == "fp32" else (0, 1, 2, 4)
     slabs = (0, 1, 2, 3) if precision == "fp32" else (0, 1, 2)
     for slab in slabs:
         for pencil in pencils:
