@@ -344,7 +344,7 @@ def main():
     for _ in range(args.warmup):
         batch.density_update()
     # kernel schedule for this grid, picked by the autotuner on this workload
-    schedule = batch.autotune(reps=2) if not args.no_autotune else []
+    schedule = batch.autotune(reps=4) if not args.no_autotune else []
 
     # timed region: the captured launch sequence replayed once per step
     t0 = time.perf_counter()
