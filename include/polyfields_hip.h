@@ -222,10 +222,16 @@ int pf_batch_set_graph(pf_batch* b, int on);
 
 /* Autotuner (no counterpart in the reference).  Times every candidate kernel schedule of
    the grid on the caller's own workload, `reps` density! evaluations each, and keeps the
-   fastest for this grid size and precision for the rest of the process.  Candidates:
-   3-D xy-slab form (0 generic, 1 compile-time 256 threads, 2 compile-time 512 threads) x
-   z-pencil form (0 generic, 1 compile-time LDS, 2 register-resident) x, in a batch, the
-   streams per lane group (1 or 2); 1-D/2-D planar kernel (0 generic, 1 compile-time).
+   fastest for this grid size and precision for the rest of the process (the four
+   fastest are re-timed in interleaved rounds).  Candidates:
+   3-D xy-slab form: 0 generic, 1 compile-time 256 threads, 2 compile-time 512 threads,
+     3 shuffle-exchange (FP32 64 x 64 planes), 4 compile-time 1024 threads (FP32 128^2);
+   z-pencil form: 0 generic, 1 compile-time LDS, 2 register-resident, 3 two threads per
+     column (FP32, 64-point lines), 4 LDS with one ky row per workgroup (64-point lines) or
+     512 threads (FP32 128-point lines), 5 LDS with 1024 threads (FP32 128-point lines);
+   in a batch, the streams per 3-D lane group (1 or 2);
+   1-D/2-D planar kernel: 0 generic, 1 compile-time (2-D: 256 threads), 2 (2-D: 512),
+     3 (2-D: 1024).
    Schedules with different transform forms round differently (about 1 ulp per step).
    sched receives {slab, pencil, planar, split} (-1: not a candidate for this grid). */
 int pf_system_autotune(pf_system* sys, int reps, int* sched, double* best_ms);
@@ -235,9 +241,10 @@ int pf_batch_autotune(pf_batch* b, int reps, int* sched, int cap, int* count);
 /* Forget every tuned schedule (built-in defaults again). */
 int pf_schedule_reset(void);
 /* Set the schedule of one grid size and precision (elem_bytes 4 or 8) by hand:
-   sched = {slab, pencil, planar, split}, -1 for the default of an entry.  Pencil 3 is the
-   two-threads-per-column form (FP32, nz = 64).  Systems already built pick it up at their
-   next launch; batches rebuild on their next membership or cell change. */
+   sched = {slab, pencil, planar, split} with the form numbers above, -1 for the default
+   of an entry (a form the grid does not have falls back to the default).  Systems already
+   built pick it up at their next launch; batches rebuild on their next membership or cell
+   change. */
 int pf_schedule_set(int elem_bytes, int nx, int ny, int nz, const int* sched);
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask /* nsys flags or NULL: all */,
