@@ -243,9 +243,8 @@ int pf_schedule_reset(void);
 /* Set the schedule of one grid size and precision (elem_bytes 4 or 8) by hand:
    sched = {slab, pencil, planar, split} with the form numbers above, -1 for the default
    of an entry (a form the grid does not have falls back to another compile-time form, or
-   the generic one).  Systems already
-   built pick it up at their next launch; batches rebuild on their next membership or cell
-   change. */
+   the generic one).  Systems already built pick it up at their next launch; batches
+   rebuild on their next membership or cell change. */
 int pf_schedule_set(int elem_bytes, int nx, int ny, int nz, const int* sched);
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask /* nsys flags or NULL: all */,
