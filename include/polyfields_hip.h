@@ -164,10 +164,10 @@ int pf_density_vjp(pf_system* sys, const double* rho_bar, const double* Q_bar, d
 /* Checkpointed propagator sweeps (K > 0: a checkpoint every K slices; -1: K = ceil(sqrt(Ns+1));
    0: full stacks, the default).  Each chain then keeps q and q+ at the checkpoints plus one
    segment of K+1 slices per direction, about 2 (Ns/K + K) slices instead of 2 (Ns+1);
-   density!, residuals!, free energies, pf_fhelm_grad and pf_density_vjp recompute segments
-   as they need them.  The reference stress formula (pf_scfstress), the propagator read-back
-   and batches need full stacks and raise.  No counterpart in the reference, which keeps
-   every slice (species/multiblock.jl:109-110). */
+   density!, residuals!, free energies, pf_scfstress (so also the DomainUpdater),
+   pf_fhelm_grad and pf_density_vjp recompute segments as they need them.  The propagator
+   read-back and batches need full stacks and raise.  No counterpart in the reference, which
+   keeps every slice (species/multiblock.jl:109-110). */
 int pf_system_set_checkpoint(pf_system* sys, int K);
 /* Set the block lengths of chain species isp to continuous values N_block[nblk]; every
    block keeps its number of contour steps and ds_b = v_b N_b / steps_b. */

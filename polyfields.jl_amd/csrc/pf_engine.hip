@@ -576,15 +576,23 @@ class Engine : public EngineBase {
         for (size_t i = 0; i < s.species.size(); ++i) {
             const Species& sp = s.species[i];
             if (sp.kind != 0) continue;
+            const SpBuf& b = sb[i];
             for (int sl = 1; sl <= sp.Ns + 1; ++sl) {
-                const int r = sb[i].row0 + sl - 1;
+                const int r = b.row0 + sl - 1;
                 StressLane<T>& L = slanes[r];
                 L.nx = nx;
                 L.ny = ny;
                 L.nz = nz;
-                L.q = sb[i].q + (size_t)(sl - 1) * n;
-                L.qc = sb[i].qc ? sb[i].qc + (size_t)(sl - 1) * n
-                                : sb[i].q + (size_t)(sp.Ns + 1 - sl) * n;
+                if (ckK) {
+                    // checkpointed: slice sl is read from the segment buffers while the
+                    // segment that owns its row (stress_seg_rows) is recomputed
+                    const size_t slot = (size_t)(sl - b.bnd[stress_seg(b, sl)]);
+                    L.q = b.q + slot * n;
+                    L.qc = b.qc + slot * n;
+                } else {
+                    L.q = b.q + (size_t)(sl - 1) * n;
+                    L.qc = b.qc ? b.qc + (size_t)(sl - 1) * n : b.q + (size_t)(sp.Ns + 1 - sl) * n;
+                }
                 L.out = r;
                 L.nparams = np;
                 L.dim = s.cell.dim;
@@ -607,36 +615,60 @@ class Engine : public EngineBase {
         slanes_cell_version = s.cell.version;
     }
 
+    // Segment of a checkpointed chain whose stress rows include slice sl: bnd[m] .. bnd[m+1]-1,
+    // the last segment also its end slice.
+    static int stress_seg(const SpBuf& b, int sl) {
+        const int nseg = (int)b.bnd.size() - 1;
+        int m = (int)(std::upper_bound(b.bnd.begin(), b.bnd.end(), sl) - b.bnd.begin()) - 1;
+        return std::min(std::max(m, 0), nseg - 1);
+    }
+
+    // stress rows [r0, r0 + cnt) into srows (planar) or the k-space partial sums (3-D)
+    void stress_launch(int r0, int cnt, hipStream_t st) {
+        const System& s = *S;
+        if (planar && 2 * lds_plane <= 160 * 1024) {
+            launch_stress_planar<T>(d_slanes + r0, cnt, srows, 2 * lds_plane, 256, st);
+            return;
+        }
+        const int ny = s.dims[1], nz = s.dims[2];
+        for (int a = r0; a < r0 + cnt; a += schunk) {
+            const int c = std::min(schunk, r0 + cnt - a);
+            launch_stress_slab<T>(d_slanes + a, c, nz, lds_plane, 256, st);
+            launch_stress_pencil<T>(d_slanes + a, c, ny, ntiles, tile, spart, 2 * lds_pencil,
+                                    256, st);
+        }
+    }
+
     void stress_rows(System& s, std::vector<double>& rows) override {
-        if (s.ckpt)
-            throw Error(-2, "the reference stress formula needs full propagator stacks; with "
-                            "checkpointed sweeps use the free-energy gradient's cell derivative");
         if (s.dirty_species || sb.size() != s.species.size()) build();
         if (slanes_cell_version != s.cell.version || (int)slanes.size() != nrows) build_stress();
         rows.assign((size_t)nrows * PF_MAXPAR, 0.0);
         if (!nrows) return;
         hipStream_t st = s.stream;
-        const bool fused = planar && 2 * lds_plane <= 160 * 1024;
-        if (fused) {
-            launch_stress_planar<T>(d_slanes, nrows, srows, 2 * lds_plane, 256, st);
-        } else {
-            const int ny = s.dims[1], nz = s.dims[2];
-            const int per = ny * ntiles;
-            for (int r0 = 0; r0 < nrows; r0 += schunk) {
-                const int cnt = std::min(schunk, nrows - r0);
-                launch_stress_slab<T>(d_slanes + r0, cnt, nz, lds_plane, 256, st);
-                launch_stress_pencil<T>(d_slanes + r0, cnt, ny, ntiles, tile, spart,
-                                        2 * lds_pencil, 256, st);
+        if (ckK) {
+            // checkpointed sweeps: each segment of q and q+ is recomputed from its
+            // checkpoints (as for Simpson) and its rows are taken before the next one
+            for (size_t i = 0; i < s.species.size(); ++i) {
+                if (s.species[i].kind != 0) continue;
+                const SpBuf& b = sb[i];
+                const int nseg = (int)b.bnd.size() - 1;
+                for (int m = 0; m < nseg; ++m) {
+                    run_segment(b.jq, m, false);
+                    run_segment(b.jc, m, false);
+                    const int lo = b.bnd[m], hi = (m == nseg - 1) ? b.bnd[m + 1] : b.bnd[m + 1] - 1;
+                    stress_launch(b.row0 + lo - 1, hi - lo + 1, st);
+                }
             }
-            launch_segsum(spart, nrows, per, PF_MAXPAR, PF_MAXPAR, srows, st);
+        } else {
+            stress_launch(0, nrows, st);
         }
+        if (!(planar && 2 * lds_plane <= 160 * 1024))
+            launch_segsum(spart, nrows, s.dims[1] * ntiles, PF_MAXPAR, PF_MAXPAR, srows, st);
         PF_HIP(hipMemcpyAsync(rows.data(), srows, sizeof(double) * nrows * PF_MAXPAR,
                               hipMemcpyDeviceToHost, st));
         PF_HIP(hipStreamSynchronize(st));
     }
 
-    // Reverse sweeps on checkpointed stacks: each adjoint lane walks the segments in its
-    // own order (q-bar from the top, q+-bar from the bottom); before a segment, q and q+
     // Reverse-sweep launches: the compile-time kernels when the size has them (and the
     // schedule does not ask for the generic ones), else the generic ones.  apper records
     // the pencil workgroups per lane, i.e. the rows of k-space partial sums.
@@ -656,6 +688,8 @@ class Engine : public EngineBase {
         apper = pp ? pp : s.dims[1] * ntiles;
     }
 
+    // Reverse sweeps on checkpointed stacks: each adjoint lane walks the segments in its
+    // own order (q-bar from the top, q+-bar from the bottom); before a segment, q and q+
     // of that segment are recomputed from their checkpoints.  At a segment edge the
     // next step's B spectrum is made by a separate launch once its input slice exists.
     void adjoint_ckpt(System& s) {
