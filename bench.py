@@ -179,10 +179,6 @@ def batched_scft(P, count, plain_sample):
     }
 
 
-# HBM bytes per lane and launch, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
-# workload (64^3 FP32, 32 lanes per launch), scaled by the counters' reading for a plain
-# float4 copy of 68 MiB (FETCH_SIZE reads half the bytes copied, WRITE_SIZE all of them):
-# slab 2 * 31197.2 + 66228.7 KiB, pencil 2 * 17123.8 + 33792.0 KiB per 32-lane launch.
 # HBM bytes per lane and launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
 # 32-lane dispatch, FETCH x 2.0 and WRITE x 1.0 from a plain copy kernel of known size),
 # per xy-slab form: 3 the shuffle-exchange kernel (default), 2 the 512-thread LDS kernel

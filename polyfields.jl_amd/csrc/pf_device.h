@@ -124,20 +124,47 @@ __device__ __forceinline__ double ksq_at(const double* G, double jx, double jy, 
     return kx * kx + ky * ky + kz * kz;
 }
 
+// Block b of step t and the slice s it reads.  q (dir > 0) reads slice s = 1 + t and is in
+// the last block with begin[b] <= s; q+ reads s = Ns + 1 - t and is in the last block with
+// begin[b] < s.  begin[] increases, so b counts the k >= 1 meeting that test: an unrolled
+// count over PF_MAXBLK entries, whose loads all go out at once (a loop over begin[] would
+// make each load wait for the previous compare, a chain of dependent scalar loads at the
+// start of every launch).
 template <typename T>
 __device__ __forceinline__ int step_block(const Lane<T>* L, int t, int* s_in) {
-    if (L->dir > 0) {
-        const int s = 1 + t;
-        *s_in = s;
-        int b = 0;
-        while (b + 1 < L->nblk && L->begin[b + 1] <= s) ++b;
-        return b;
-    }
-    const int s = L->nsteps + 1 - t;
+    // every field is read unconditionally (a short-circuit test would make the loads
+    // conditional, hence sequential)
+    const int dir = L->dir, ns = L->nsteps, nb = L->nblk;
+    int bg[PF_MAXBLK];
+#pragma unroll
+    for (int k = 1; k < PF_MAXBLK; ++k) bg[k] = L->begin[k];
+    const bool fwd = dir > 0;
+    const int s = fwd ? 1 + t : ns + 1 - t;
     *s_in = s;
-    int b = L->nblk - 1;
-    while (b > 0 && L->begin[b] + 1 > s) --b;
+    const int lim = fwd ? s : s - 1;
+    int b = 0;
+#pragma unroll
+    for (int k = 1; k < PF_MAXBLK; ++k) b += (int)(k < nb) & (int)(bg[k] <= lim);
     return b;
+}
+
+// a[b] of a per-block lane table with every entry loaded up front and b selecting among
+// them (no load that waits for b)
+template <typename P>
+__device__ __forceinline__ P pick_blk(const P (&a)[PF_MAXBLK], int b) {
+    static_assert(sizeof(P) == 8, "per-block entries of 8 bytes");
+    static_assert(PF_MAXBLK == 8, "one operand per block below");
+    unsigned long long e[PF_MAXBLK];
+#pragma unroll
+    for (int k = 0; k < PF_MAXBLK; ++k) e[k] = __builtin_bit_cast(unsigned long long, a[k]);
+    // all eight loads go out together; the empty asm keeps the select below from becoming
+    // a dependent load of a[b]
+    asm volatile("" : "+s"(e[0]), "+s"(e[1]), "+s"(e[2]), "+s"(e[3]), "+s"(e[4]), "+s"(e[5]),
+                 "+s"(e[6]), "+s"(e[7]));
+    unsigned long long r = e[0];
+#pragma unroll
+    for (int k = 1; k < PF_MAXBLK; ++k) r = (b == k) ? e[k] : r;
+    return __builtin_bit_cast(P, r);
 }
 
 __device__ __forceinline__ double dksq_at(const double* dGG, int np, int k, int dim, double j0,

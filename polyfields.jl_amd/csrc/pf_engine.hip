@@ -26,6 +26,16 @@ static FftPlan make_plan(int n) {
     return p;
 }
 
+// Lanes [off, off + cnt) of a launch at step t: 1 + the last one still running (nsteps >
+// t).  Lanes past it are finished and are not launched; a finished lane before it is
+// launched and skips its work in the kernel.
+template <typename T>
+static int live_lanes(const std::vector<Lane<T>>& v, int off, int cnt, int t) {
+    for (int i = cnt; i > 0; --i)
+        if (v[(size_t)(off + i - 1)].nsteps > t) return i;
+    return 0;
+}
+
 template <typename T>
 static std::vector<cpx<T>> twiddles(int n) {
     std::vector<cpx<T>> t(std::max(n, 1));
@@ -525,13 +535,14 @@ class Engine : public EngineBase {
                         launch_slab<T>(d_lanes, nl, nz, SLAB_FWD, 0, lds_plane, 256, st);
                 });
                 for (int t = 0; t < maxNs; ++t) {
+                    const int nla = live_lanes(lanes, 0, nl, t);
                     timed(1, [&] {
-                        if (!fixed || !fixed_pencil<T>(d_lanes, nl, s.dims[0], ny, nz, ntiles, tile, t, st))
-                            launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st);
+                        if (!fixed || !fixed_pencil<T>(d_lanes, nla, s.dims[0], ny, nz, ntiles, tile, t, st))
+                            launch_pencil<T>(d_lanes, nla, ny, ntiles, tile, t, lds_pencil, 256, st);
                     });
                     timed(0, [&] {
-                        if (!fixed || !fixed_slab<T>(d_lanes, nl, s.dims[0], ny, nz, SLAB_INVFWD, t, st))
-                            launch_slab<T>(d_lanes, nl, nz, SLAB_INVFWD, t, lds_plane, 256, st);
+                        if (!fixed || !fixed_slab<T>(d_lanes, nla, s.dims[0], ny, nz, SLAB_INVFWD, t, st))
+                            launch_slab<T>(d_lanes, nla, nz, SLAB_INVFWD, t, lds_plane, 256, st);
                     });
                 }
             }
@@ -1154,13 +1165,14 @@ class BatchEngine : public BatchEngineBase {
                         launch_slab<T>(dl, gr.count, gr.nz, SLAB_FWD, 0, gr.lds_plane, 256, gr.st);
                 });
                 for (int t = 0; t < gr.maxNs; ++t) {
-                    timed(1, gr.st, gr.count, [&] {
-                        if (!fixed_pencil<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, gr.ntiles, gr.tile, t, gr.st))
-                            launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st);
+                    const int nla = live_lanes(lanes, gr.off, gr.count, t);
+                    timed(1, gr.st, nla, [&] {
+                        if (!fixed_pencil<T>(dl, nla, gr.nx, gr.ny, gr.nz, gr.ntiles, gr.tile, t, gr.st))
+                            launch_pencil<T>(dl, nla, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st);
                     });
-                    timed(0, gr.st, gr.count, [&] {
-                        if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_INVFWD, t, gr.st))
-                            launch_slab<T>(dl, gr.count, gr.nz, SLAB_INVFWD, t, gr.lds_plane, 256, gr.st);
+                    timed(0, gr.st, nla, [&] {
+                        if (!fixed_slab<T>(dl, nla, gr.nx, gr.ny, gr.nz, SLAB_INVFWD, t, gr.st))
+                            launch_slab<T>(dl, nla, gr.nz, SLAB_INVFWD, t, gr.lds_plane, 256, gr.st);
                     });
                 }
             }

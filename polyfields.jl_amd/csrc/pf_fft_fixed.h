@@ -191,12 +191,31 @@ __device__ __forceinline__ void cplane_inverse(cpx<T>* S, const cpx<T>* twx, con
     cfft<T, M, NY, 1, LD, NT, true, 2>(S, twx);
 }
 
+// COUNT values from global memory to LDS.  Loads go out in groups of up to 8 per thread
+// before any of their stores.  A thread past the end copies element i % COUNT again (the
+// same value to the same place, and distinct places across a wavefront, so the stores
+// do not collide in one bank): no branch, so the compiler cannot move a load into a
+// conditional block behind the waits for the others.
 template <typename T, int COUNT, int NT>
 __device__ __forceinline__ void lds_copy_in(cpx<T>* dst, const cpx<T>* __restrict__ src) {
     const auto g = gp(src);
+    constexpr int IT = (COUNT + NT - 1) / NT, CH = 8;
 #pragma unroll
-    for (int i0 = 0; i0 < COUNT; i0 += NT)
-        if (i0 + (int)threadIdx.x < COUNT) dst[i0 + threadIdx.x] = g[i0 + threadIdx.x];
+    for (int c0 = 0; c0 < IT; c0 += CH) {
+        cpx<T> v[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c0 + c < IT) {
+                const int i = (c0 + c) * NT + (int)threadIdx.x;
+                v[c] = g[i < COUNT ? i : i % COUNT];
+            }
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c0 + c < IT) {
+                const int i = (c0 + c) * NT + (int)threadIdx.x;
+                dst[i < COUNT ? i : i % COUNT] = v[c];
+            }
+    }
 }
 
 // packed real plane (pairs, M per row, NY rows) <-> LDS rows of stride M+1

@@ -70,6 +70,20 @@ __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ la
     }
 }
 
+// forward xy transform of the plane in LDS and its half spectrum out to kb
+template <typename T, int NX, int NY, int NT, bool PPS, typename K>
+__device__ __forceinline__ void slab_forward_out(cpx<T>* S, cpx<T>* S2, const cpx<T>* TWX,
+                                                 const cpx<T>* TWY, K kb) {
+    constexpr int KPLANE = (NX / 2 + 1) * NY;
+    if constexpr (PPS)
+        cplane_forward_pp<T, NX, NY, NT>(S, S2, TWX, TWY);
+    else
+        cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+#pragma unroll
+    for (int i0 = 0; i0 < KPLANE; i0 += NT)
+        if (i0 + (int)threadIdx.x < KPLANE) kb[i0 + threadIdx.x] = S[i0 + threadIdx.x];
+}
+
 template <typename T, int NX, int NY, int MODE, int NT>
 __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
     constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
@@ -80,28 +94,45 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
     __shared__ cpx<T> S2[PPS ? KPLANE : 1];
     __shared__ cpx<T> TWX[NX];
     __shared__ cpx<T> TWY[NY];
-    const int lane = blockIdx.x / nz;
-    const int z = blockIdx.x - lane * nz;
-    const Lane<T>* L = lanes + lane;
-    if (t >= L->nsteps) return;
+    // grid (nz, lanes): plane z of lane blockIdx.y
+    const int z = blockIdx.x;
+    const Lane<T>* L = lanes + blockIdx.y;
     const size_t n = (size_t)PLANE * nz;
-    const auto kb = gp(L->kbuf + (size_t)z * KPLANE);
-    lds_copy_in<T, NX, NT>(TWX, L->twx);
-    lds_copy_in<T, NY, NT>(TWY, L->twy);
     int s_in;
-    int b = step_block(L, t, &s_in);
     if constexpr (MODE == SLAB_INVFWD) {
-        // the LW rows of this plane are loaded first, so that their latency overlaps the
-        // inverse transforms
+        // Every lane field the first loads need comes in one round of scalar loads (the
+        // empty asm keeps them ahead of every branch).  The LW rows of this plane are
+        // loaded first, so that their latency overlaps the inverse transforms, then the
+        // spectrum plane and the twiddles.  A finished lane (t >= Ns) loads its (valid)
+        // buffers too and returns before its first barrier.
+        unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf),
+                           txr = __builtin_bit_cast(unsigned long long, L->twx),
+                           tyr = __builtin_bit_cast(unsigned long long, L->twy);
+        int s2;
+        const int b = step_block(L, t, &s_in);
+        const int nsteps = L->nsteps;
+        const bool more = t + 1 < nsteps;
+        const int bn = step_block(L, t + 1, &s2), b2 = more ? bn : b;
+        const T* const lwp = pick_blk(L->lw, b);
+        unsigned long long lw2raw = __builtin_bit_cast(unsigned long long, pick_blk(L->lw, b2));
+        asm volatile("" : "+s"(kraw), "+s"(txr), "+s"(tyr), "+s"(lw2raw));
+        cpx<T>* const kbp = __builtin_bit_cast(cpx<T>*, kraw) + (size_t)z * KPLANE;
+        const auto kb = gp(kbp);
         constexpr int NP = (NX / 2) * NY, PER = (NP + NT - 1) / NT;
-        const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE));
+        const auto lw = gp(reinterpret_cast<const cpx<T>*>(lwp + (size_t)z * PLANE));
         cpx<T> wr[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = k * NT + threadIdx.x;
             if (i < NP) wr[k] = lw[i];
         }
-        lds_copy_in<T, KPLANE, NT>(S, L->kbuf + (size_t)z * KPLANE);
+        lds_copy_in<T, KPLANE, NT>(S, kbp);
+        lds_copy_in<T, NX, NT>(TWX, __builtin_bit_cast(const cpx<T>*, txr));
+        lds_copy_in<T, NY, NT>(TWY, __builtin_bit_cast(const cpx<T>*, tyr));
+        // the LW values count as used here, so their loads stay ahead of the exit test
+#pragma unroll
+        for (int k = 0; k < PER; ++k) asm volatile("" : "+v"(wr[k].x), "+v"(wr[k].y));
+        if (t >= nsteps) return;
         __syncthreads();
         if constexpr (PPS)
             cplane_inverse_pp<T, NX, NY, NT>(S, S2, TWX, TWY);
@@ -110,10 +141,8 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
         const T scale = T(2.0 / (double)n);
         const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
                                                        (size_t)z * PLANE));
-        const bool more = t + 1 < L->nsteps;
-        int s2;
-        const int b2 = more ? step_block(L, t + 1, &s2) : b;
-        const auto lw2 = gp(reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * PLANE));
+        const auto lw2 = gp(reinterpret_cast<const cpx<T>*>(__builtin_bit_cast(const T*, lw2raw) +
+                                                            (size_t)z * PLANE));
         if (b2 == b) {
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
@@ -141,7 +170,15 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
             }
         }
         if (!more) return;
+        __syncthreads();
+        slab_forward_out<T, NX, NY, NT, PPS>(S, S2, TWX, TWY, kb);
     } else {
+        // the first step: LW q_1 from the stack (one launch per sweep)
+        if (t >= L->nsteps) return;
+        const auto kb = gp(L->kbuf + (size_t)z * KPLANE);
+        lds_copy_in<T, NX, NT>(TWX, L->twx);
+        lds_copy_in<T, NY, NT>(TWY, L->twy);
+        const int b = step_block(L, t, &s_in);
         const auto qin = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
                                                             (size_t)z * PLANE));
         const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE));
@@ -149,15 +186,9 @@ __global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lane
             const cpx<T> q = qin[i], w = lw[i];
             S[o] = {q.x * w.x, q.y * w.y};
         });
+        __syncthreads();
+        slab_forward_out<T, NX, NY, NT, PPS>(S, S2, TWX, TWY, kb);
     }
-    __syncthreads();
-    if constexpr (PPS)
-        cplane_forward_pp<T, NX, NY, NT>(S, S2, TWX, TWY);
-    else
-        cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
-#pragma unroll
-    for (int i0 = 0; i0 < KPLANE; i0 += NT)
-        if (i0 + (int)threadIdx.x < KPLANE) kb[i0 + threadIdx.x] = S[i0 + threadIdx.x];
 }
 
 // xy-slab step for FP32 64 x 64 planes (steps after the first), 256 threads per plane,
@@ -228,31 +259,58 @@ __global__ void __launch_bounds__(256, 4) k_slab_sx(const Lane<float>* __restric
     __shared__ C NQ[64];       // column kx = 32 of the input spectrum
     __shared__ C W[4 * 576];   // per-wavefront transpose regions
     __shared__ C TW[64];       // e^{-2 pi i m / 64}
-    const int lane = blockIdx.x / nz;
-    const int z = blockIdx.x - lane * nz;
-    const Lane<float>* L = lanes + lane;
-    if (t >= L->nsteps) return;
+    // grid (nz, lanes): plane z of lane blockIdx.y.  Every lane field the first loads need
+    // comes from one round of scalar loads; a finished lane (t >= Ns) loads its (valid)
+    // buffers like the others and returns before its first barrier.
+    const int z = blockIdx.x;
+    const Lane<float>* L = lanes + blockIdx.y;
     const size_t n = (size_t)PLANE * nz;
     const int tid = threadIdx.x, w = tid >> 6, ln = tid & 63;
+    unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf),
+                       twraw = __builtin_bit_cast(unsigned long long, L->twy);
     int s_in, s2;
     const int b = step_block(L, t, &s_in);
-    const bool more = t + 1 < L->nsteps;
-    const int b2 = more ? step_block(L, t + 1, &s2) : b;
+    const int nsteps = L->nsteps;
+    const bool more = t + 1 < nsteps;
+    const int bn = step_block(L, t + 1, &s2), b2 = more ? bn : b;   // a select, not a branch
+    const float* const lwp = pick_blk(L->lw, b);
+    unsigned long long lw2raw = __builtin_bit_cast(unsigned long long, pick_blk(L->lw, b2));
+    // these pointers are used only from here on, so their loads join the round above
+    asm volatile("" : "+s"(kraw), "+s"(twraw), "+s"(lw2raw));
+    const auto kb = gp(__builtin_bit_cast(cpx<float>*, kraw) + (size_t)z * KPLANE);
+    const cpx<float>* const twy = __builtin_bit_cast(const cpx<float>*, twraw);
     // row-phase position and its LW values, loaded first
     const int rho = ln >> 2, lp = ln & 3, yr = 16 * w + rho;
     const unsigned roff = (unsigned)(yr * 64 + 4 * lp) * 4u;
-    const sx::PlaneBuf lwb(L->lw[b] + (size_t)z * PLANE, PLANE * 4);
+    const sx::PlaneBuf lwb(lwp + (size_t)z * PLANE, PLANE * 4);
     sx::f4 wr[4];
 #pragma unroll
     for (int m2 = 0; m2 < 4; ++m2) wr[m2] = lwb.load(roff + 64u * m2);
-    if (tid < 64) TW[tid] = gp(L->twy)[tid];
-    const auto kb = gp(L->kbuf + (size_t)z * KPLANE);
+    // the spectrum plane, column kx = 32 and the twiddles: every load issued before the
+    // first LDS store waits for them (loads complete in order)
+    C kv[8];
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const int i = s * 256 + tid, ky = i >> 5, kx = i & 31;
-        P[sx::swz(kx, ky)] = kb[ky * 33 + kx];
+        const int i = s * 256 + tid;
+        kv[s] = kb[(i >> 5) * 33 + (i & 31)];
     }
-    if (tid < 64) NQ[tid] = kb[tid * 33 + 32];
+    const C nq = kb[ln * 33 + 32], tw = gp(twy)[ln];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int i = s * 256 + tid;
+        P[sx::swz(i & 31, i >> 5)] = kv[s];
+    }
+    // every wavefront stores the same 64 values (a condition here would move the two loads
+    // behind the others' completion)
+    NQ[ln] = nq;
+    TW[ln] = tw;
+    // the LW values count as used here, so their loads stay ahead of the exit test (they
+    // were issued before the spectrum loads and have arrived by now)
+    asm volatile("" : "+v"(wr[0].x), "+v"(wr[0].y), "+v"(wr[0].z), "+v"(wr[0].w),
+                 "+v"(wr[1].x), "+v"(wr[1].y), "+v"(wr[1].z), "+v"(wr[1].w),
+                 "+v"(wr[2].x), "+v"(wr[2].y), "+v"(wr[2].z), "+v"(wr[2].w),
+                 "+v"(wr[3].x), "+v"(wr[3].y), "+v"(wr[3].z), "+v"(wr[3].w));
+    if (t >= nsteps) return;
     __syncthreads();
 
     C* const Wr = W + 576 * w;
@@ -322,7 +380,8 @@ __global__ void __launch_bounds__(256, 4) k_slab_sx(const Lane<float>* __restric
         const float scale = (float)(2.0 / (double)n);
         const sx::PlaneBuf qb(L->stack + (size_t)(s_in + L->dir - 1) * n + (size_t)z * PLANE,
                               PLANE * 4);
-        const sx::PlaneBuf lw2b(L->lw[b2] + (size_t)z * PLANE, PLANE * 4);
+        const sx::PlaneBuf lw2b(__builtin_bit_cast(const float*, lw2raw) + (size_t)z * PLANE,
+                                PLANE * 4);
 #pragma unroll
         for (int m2 = 0; m2 < 4; ++m2) {
             const sx::f4 wv = wr[m2];
@@ -401,24 +460,41 @@ __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ la
     constexpr int C = KB * NXH;
     __shared__ cpx<T> P[NZ * C];
     __shared__ cpx<T> TWZ[NZ];
-    const int per = ny / KB;
-    const int lane = blockIdx.x / per;
-    const int ky0 = (blockIdx.x - lane * per) * KB;
-    const Lane<T>* L = lanes + lane;
-    if (t >= L->nsteps) return;
-    lds_copy_in<T, NZ, NT>(TWZ, L->twz);
-    int s_in;
-    const int b = step_block(L, t, &s_in);
-    const auto kb = gp(L->kbuf + (size_t)ky0 * NXH);
+    // grid (ny / KB, lanes).  The lane fields come in one round of scalar loads (the empty
+    // asm keeps them ahead of every branch); a finished lane (t >= Ns) loads its (valid)
+    // rows too and returns before its first barrier.
+    const Lane<T>* L = lanes + blockIdx.y;
+    const int ky0 = blockIdx.x * KB;
+    unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf),
+                       tzr = __builtin_bit_cast(unsigned long long, L->twz);
+    int nsteps = L->nsteps, s_in;
+    int b = __builtin_amdgcn_readfirstlane(step_block(L, t, &s_in));
+    asm volatile("" : "+s"(kraw), "+s"(tzr), "+s"(nsteps), "+s"(b));
+    const auto kb = gp(__builtin_bit_cast(cpx<T>*, kraw) + (size_t)ky0 * NXH);
     const size_t zstride = (size_t)ny * NXH;
+    // the KB ky rows of every kz into LDS: loads in groups of 8 per thread ahead of their
+    // stores, a thread past the end repeating element i % CNT (no branch)
+    constexpr int CNT = NZ * C, IT = (CNT + NT - 1) / NT, CH = 8;
 #pragma unroll
-    for (int i0 = 0; i0 < NZ * C; i0 += NT) {
-        const int i = i0 + threadIdx.x;
-        if (i < NZ * C) P[i] = kb[(i / C) * zstride + (i % C)];
+    for (int c0 = 0; c0 < IT; c0 += CH) {
+        cpx<T> v[CH];
+        int o[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c0 + c < IT) {
+                const int i0 = (c0 + c) * NT + (int)threadIdx.x, i = i0 < CNT ? i0 : i0 % CNT;
+                o[c] = i;
+                v[c] = kb[(i / C) * zstride + (i % C)];
+            }
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+            if (c0 + c < IT) P[o[c]] = v[c];
     }
+    lds_copy_in<T, NZ, NT>(TWZ, __builtin_bit_cast(const cpx<T>*, tzr));
+    if (t >= nsteps) return;
     __syncthreads();
     cfft<T, NZ, C, C, 1, NT, false, 1>(P, TWZ);
-    const double mc = -L->ldc[b];
+    const double mc = -pick_blk(L->ldc, b);
     const double* G = L->G;
 #pragma unroll
     for (int i0 = 0; i0 < NZ * C; i0 += NT) {
@@ -557,24 +633,29 @@ struct SpecBuf {
 // transformed, multiplied by LD, transformed back and written, with no LDS and no barrier.
 template <typename T, int NZ, int NT>
 __global__ void __launch_bounds__(NT) k_pencil_reg(const Lane<T>* __restrict__ lanes, int ny,
-                                                   int per, int t) {
+                                                   int nxh, int t) {
     constexpr int R1 = RegSplit<NZ>::R1, R2 = RegSplit<NZ>::R2;
-    const int lane = blockIdx.x / per;
-    const int c = (blockIdx.x - lane * per) * NT + threadIdx.x;
-    const Lane<T>* L = lanes + lane;
-    if (t >= L->nsteps) return;
-    const int nxh = (L->nx >> 1) + 1, ncol = nxh * ny;
+    // grid (column blocks, lanes).  The column loads wait for one round of scalar loads of
+    // the lane record (the empty asm keeps those loads ahead of every branch).  A finished
+    // lane (t >= Ns) runs through and its stores go to an empty buffer range, so no branch
+    // lets the compiler move the column loads after the test.
+    const Lane<T>* L = lanes + blockIdx.y;
+    unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf);
+    int nsteps = L->nsteps, s_in;
+    int b = __builtin_amdgcn_readfirstlane(step_block(L, t, &s_in));
+    asm volatile("" : "+s"(kraw), "+s"(nsteps), "+s"(b));
+    cpx<T>* const kbp = __builtin_bit_cast(cpx<T>*, kraw);
+    const int c = blockIdx.x * NT + threadIdx.x, ncol = nxh * ny;
     if (c >= ncol) return;
-    int s_in;
-    const int b = step_block(L, t, &s_in);
     const unsigned row = (unsigned)ncol * (unsigned)sizeof(cpx<T>);
-    const SpecBuf<T> kb(L->kbuf, row * (unsigned)NZ);
+    const SpecBuf<T> kb(kbp, row * (unsigned)NZ);
+    const SpecBuf<T> kout(kbp, t < nsteps ? row * (unsigned)NZ : 0u);
     const unsigned off = (unsigned)c * (unsigned)sizeof(cpx<T>);
     cpx<T> v[NZ];
 #pragma unroll
     for (int z = 0; z < NZ; ++z) v[z] = kb.load(off, (unsigned)z * row);
     fft_reg<T, NZ, R1, R2, false>(v);
-    const double mc = -L->ldc[b];
+    const double mc = -pick_blk(L->ldc, b);
     const double* G = L->G;
     const int ky = c / nxh, kx = c - ky * nxh;
     const double jx = (double)kx, jy = (double)fold(ky, ny);
@@ -602,7 +683,7 @@ __global__ void __launch_bounds__(NT) k_pencil_reg(const Lane<T>* __restrict__ l
     }
     fft_reg<T, NZ, R1, R2, true>(v);
 #pragma unroll
-    for (int z = 0; z < NZ; ++z) kb.store(v[z], off, (unsigned)z * row);
+    for (int z = 0; z < NZ; ++z) kout.store(v[z], off, (unsigned)z * row);
 }
 
 // z-pencil pass with two threads per column (FP32, NZ = 64): lane pair (2c, 2c+1) holds
@@ -747,10 +828,10 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
 #define PF_SL1(X, Y, NT)                                                                    \
     if (nx == X && ny == Y) {                                                               \
         if (mode == SLAB_FWD)                                                               \
-            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_FWD, NT>), dim3(nl * nz), dim3(NT), 0, \
+            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_FWD, NT>), dim3(nz, nl), dim3(NT), 0, \
                                st, d, nz, t);                                               \
         else                                                                                \
-            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_INVFWD, NT>), dim3(nl * nz), dim3(NT), \
+            hipLaunchKernelGGL((k_slab_fx<T, X, Y, SLAB_INVFWD, NT>), dim3(nz, nl), dim3(NT), \
                                0, st, d, nz, t);                                            \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
@@ -762,7 +843,7 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
         // FP32 64 x 64 planes, steps after the first: the shuffle-exchange form by default
         // (measured on MI355X: equal at 32 systems per batch, +7% for one system)
         if ((sk == 3 || sk < 0) && nx == 64 && ny == 64 && mode == SLAB_INVFWD) {
-            hipLaunchKernelGGL(k_slab_sx, dim3(nl * nz), dim3(256), 0, st, d, nz, t);
+            hipLaunchKernelGGL(k_slab_sx, dim3(nz, nl), dim3(256), 0, st, d, nz, t);
             PF_HIP(hipGetLastError());
             return true;
         }
@@ -789,7 +870,7 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
     if (pk == 0) return false;
 #define PF_PE(X, Z, KB)                                                                     \
     if (nx == X && nz == Z && ny % KB == 0) {                                               \
-        hipLaunchKernelGGL((k_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(nl * (ny / KB)),    \
+        hipLaunchKernelGGL((k_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(ny / KB, nl),      \
                            dim3(256), 0, st, d, ny, t);                                     \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
@@ -809,8 +890,8 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
 #define PF_PR(Z)                                                                            \
         if (nz == Z) {                                                                      \
             const int per = (ncol + 63) / 64;                                               \
-            hipLaunchKernelGGL((k_pencil_reg<T, Z, 64>), dim3(nl * per), dim3(64), 0, st, d,  \
-                               ny, per, t);                                                 \
+            hipLaunchKernelGGL((k_pencil_reg<T, Z, 64>), dim3(per, nl), dim3(64), 0, st, d,   \
+                               ny, nx / 2 + 1, t);                                          \
             PF_HIP(hipGetLastError());                                                      \
             return true;                                                                    \
         }
@@ -831,7 +912,7 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
         // 128-point lines: the LDS pencil with 512 or 1024 threads
 #define PF_PE_NT(NT)                                                                        \
         if (nx == 128 && nz == 128) {                                                       \
-            hipLaunchKernelGGL((k_pencil_fx<T, 128, 65, 1, NT>), dim3(nl * ny), dim3(NT), 0,   \
+            hipLaunchKernelGGL((k_pencil_fx<T, 128, 65, 1, NT>), dim3(ny, nl), dim3(NT), 0,    \
                                st, d, ny, t);                                               \
             PF_HIP(hipGetLastError());                                                      \
             return true;                                                                    \
