@@ -268,16 +268,20 @@ __global__ void __launch_bounds__(NT) k_adj_slab_fx(const Lane<T>* __restrict__ 
     __shared__ cpx<T> S[KPLANE];
     __shared__ cpx<T> TWX[NX];
     __shared__ cpx<T> TWY[NY];
-    const int lane = blockIdx.x / nz;
-    const int z = blockIdx.x - lane * nz;
-    const Lane<T>* L = lanes + lane;
+    // grid (nz, lanes): plane z of lane blockIdx.y
+    const int z = blockIdx.x;
+    const Lane<T>* L = lanes + blockIdx.y;
     const int NS = L->nsteps;
     if (t >= NS) return;
     const size_t n = (size_t)PLANE * nz, off = (size_t)z * PLANE;
     cpx<T>* kA = L->kbuf + (size_t)z * KPLANE;
     cpx<T>* kB = L->kb2 + (size_t)z * KPLANE;
-    lds_copy_in<T, NX, NT>(TWX, L->twx);
-    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    // the twiddles are copied after the first plane's loads are out (before the first
+    // barrier), so that those loads do not wait behind them
+    auto twiddles_in = [&] {
+        lds_copy_in<T, NX, NT>(TWX, L->twx);
+        lds_copy_in<T, NY, NT>(TWY, L->twy);
+    };
     int s_in;
     int b = step_block(L, NS - 1 - t, &s_in);
     auto to_spectrum = [&](cpx<T>* kb) {
@@ -297,6 +301,7 @@ __global__ void __launch_bounds__(NT) k_adj_slab_fx(const Lane<T>* __restrict__ 
     };
     if constexpr (MODE == 2) {
         lw_times_fwd(s_in, b);
+        twiddles_in();
         to_spectrum(kB);
         return;
     }
@@ -316,8 +321,10 @@ __global__ void __launch_bounds__(NT) k_adj_slab_fx(const Lane<T>* __restrict__ 
             wb[r0 + 1] += wc * a1 * (double)a.y;
             S[o] = {(T)a0 * w.x, (T)a1 * w.y};
         });
+        twiddles_in();
     } else {
         lds_copy_in<T, KPLANE, NT>(S, kA);
+        twiddles_in();
         __syncthreads();
         cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
         const T scale = T(2.0 / (double)n);
@@ -375,14 +382,12 @@ __global__ void __launch_bounds__(NT) k_adj_pencil_fx(const Lane<T>* __restrict_
     __shared__ cpx<T> PA[NZ * C];
     __shared__ cpx<T> PB[NZ * C];
     __shared__ cpx<T> TWZ[NZ];
-    const int per = ny / KB;
-    const int lane = blockIdx.x / per;
-    const int r = blockIdx.x - lane * per;
+    // grid (ny / KB, lanes); partial-sum row r = blockIdx.x
+    const int r = blockIdx.x;
     const int ky0 = r * KB;
-    const Lane<T>* L = lanes + lane;
+    const Lane<T>* L = lanes + blockIdx.y;
     const int NS = L->nsteps;
     if (t >= NS) return;
-    lds_copy_in<T, NZ, NT>(TWZ, L->twz);
     int s_in;
     const int b = step_block(L, NS - 1 - t, &s_in);
     constexpr int M = NXH - 1;
@@ -398,6 +403,7 @@ __global__ void __launch_bounds__(NT) k_adj_pencil_fx(const Lane<T>* __restrict_
             PB[i] = kB[g];
         }
     }
+    lds_copy_in<T, NZ, NT>(TWZ, L->twz);
     __syncthreads();
     cfft<T, NZ, C, C, 1, NT, false, 1>(PA, TWZ);
     cfft<T, NZ, C, C, 1, NT, false, 1>(PB, TWZ);
@@ -448,13 +454,13 @@ bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, 
 #define PF_AS(X, Y)                                                                         \
     if (nx == X && ny == Y) {                                                               \
         if (mode == 0)                                                                      \
-            hipLaunchKernelGGL((k_adj_slab_fx<T, 0, X, Y, 256>), dim3(nl * nz), dim3(256), 0, \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 0, X, Y, 256>), dim3(nz, nl), dim3(256), 0, \
                                st, d, nz, t, nextB);                                        \
         else if (mode == 1)                                                                 \
-            hipLaunchKernelGGL((k_adj_slab_fx<T, 1, X, Y, 256>), dim3(nl * nz), dim3(256), 0, \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 1, X, Y, 256>), dim3(nz, nl), dim3(256), 0, \
                                st, d, nz, t, nextB);                                        \
         else                                                                                \
-            hipLaunchKernelGGL((k_adj_slab_fx<T, 2, X, Y, 256>), dim3(nl * nz), dim3(256), 0, \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 2, X, Y, 256>), dim3(nz, nl), dim3(256), 0, \
                                st, d, nz, t, 1);                                            \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
@@ -470,7 +476,7 @@ int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hi
     if (schedule(sizeof(T), nx, ny, nz).pencil == 0) return 0;
 #define PF_AP(X, Z, KB)                                                                     \
     if (nx == X && nz == Z && ny % KB == 0) {                                               \
-        hipLaunchKernelGGL((k_adj_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(nl * (ny / KB)),  \
+        hipLaunchKernelGGL((k_adj_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(ny / KB, nl),    \
                            dim3(256), 0, st, d, ny, t);                                     \
         PF_HIP(hipGetLastError());                                                          \
         return ny / KB;                                                                     \
