@@ -182,9 +182,9 @@ def batched_scft(P, count, plain_sample):
 # HBM bytes per lane and launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
 # 32-lane dispatch, FETCH x 2.0 and WRITE x 1.0 from a plain copy kernel of known size),
 # per xy-slab form: 3 the shuffle-exchange kernel (default), 2 the 512-thread LDS kernel
-TRAFFIC_PER_LANE = {("xy-slab pass", 3): (2 * 31202.0 + 66292.1) * 1024 / 32,
-                    ("xy-slab pass", 2): (2 * 31194.1 + 66228.7) * 1024 / 32,
-                    ("z-pencil pass", None): (2 * 17091.6 + 33792.0) * 1024 / 32}
+TRAFFIC_PER_LANE = {("xy-slab pass", 3): (2 * 31170.5 + 66327.6) * 1024 / 32,
+                    ("xy-slab pass", 2): (2 * 31193.8 + 66228.7) * 1024 / 32,
+                    ("z-pencil pass", None): (2 * 17091.8 + 33792.0) * 1024 / 32}
 
 
 def traffic_per_launch(kernel, args, slab_lanes, pen_lanes, slab_form):
@@ -313,6 +313,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--sweep-instances", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--schedule", default=None, metavar="SLAB,PENCIL",
+                    help="density64: fix the 64^3 slab and pencil kernel forms (the numbers "
+                         "of polyfields_hip.schedule_set) instead of autotuning")
     ap.add_argument("--workload", default="density64", choices=["density64", "blend128", "grad64"],
                     help="density64: the headline (64^3 batch); blend128 / grad64: the large "
                          "blend and the free-energy gradient, one JSON line each")
@@ -329,6 +332,12 @@ def main():
         print(json.dumps(fn(P, args)))
         return
 
+    fixed_schedule = []
+    if args.schedule:
+        slab, pencil = (int(x) for x in args.schedule.split(","))
+        P.schedule_set(args.precision, (GRID, GRID, GRID), slab=slab, pencil=pencil)
+        fixed_schedule = [{"nx": GRID, "ny": GRID, "nz": GRID, "slab": slab, "pencil": pencil}]
+        args.no_autotune = True
     systems = []
     for k in range(args.instances):
         s, ch = build(P, args.precision, args.seed + k)
@@ -340,7 +349,7 @@ def main():
     for _ in range(args.warmup):
         batch.density_update()
     # kernel schedule for this grid, picked by the autotuner on this workload
-    schedule = batch.autotune(reps=4) if not args.no_autotune else []
+    schedule = batch.autotune(reps=4) if not args.no_autotune else fixed_schedule
 
     # timed region: the captured launch sequence replayed once per step
     t0 = time.perf_counter()
@@ -348,8 +357,9 @@ def main():
         batch.density_update()
     dt = time.perf_counter() - t0
     # kernel-level figures from a separate pass: HIP events on the launching streams
-    # bracket one launch in 8; the launches then run eagerly and one stream at a time, so
-    # each timed launch has the device to itself, as in a kernel trace
+    # bracket one launch in 8, run 4 times back to back (the events' overhead spread over
+    # the repeats); the launches run eagerly and one stream at a time, so each timed
+    # launch has the device to itself.  The pass's densities are not used.
     batch.timing(True)
     for _ in range(max(2, args.steps // 2)):
         batch.density_update()

@@ -255,8 +255,11 @@ int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches)
    groups of 4 or more lanes are launched as two halves on two streams). */
 int pf_batch_timing_lanes(pf_batch* b, int kind, double* lanes);
 
-/* Measurement: per-launch HIP-event timing of the propagator kernels on the system's
-   stream.  kind: 0 xy-slab pass, 1 z-pencil pass, 2 whole-plane (1-D/2-D) sweep. */
+/* Measurement: HIP-event timing of the propagator kernels on the system's stream.  kind:
+   0 xy-slab pass, 1 z-pencil pass, 2 whole-plane (1-D/2-D) sweep.  One launch in 8 of each
+   kind runs 4 times back to back between two events; *launches counts every timed run.
+   The repeats act on their own output, so densities computed with timing on are not
+   meaningful (batches alike). */
 int pf_system_timing(pf_system* sys, int on);
 int pf_system_timing_read(pf_system* sys, int kind, double* total_ms, int* launches);
 

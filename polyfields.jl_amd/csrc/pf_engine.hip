@@ -26,6 +26,9 @@ static FftPlan make_plan(int n) {
     return p;
 }
 
+// launches per event pair in timing mode
+constexpr int kTimingReps = 4;
+
 // Lanes [off, off + cnt) of a launch at step t: 1 + the last one still running (nsteps >
 // t).  Lanes past it are finished and are not launched; a finished lane before it is
 // launched and skips its work in the kernel.
@@ -418,15 +421,19 @@ class Engine : public EngineBase {
         }
         return evpool[evnext++];
     }
+    // timing mode: one launch in 8 of each kind runs kTimingReps times back to back between
+    // two events (the events' own overhead is spread over the repeats); the repeats redo
+    // the launch on its own output, so the sweep's results are not meaningful then
+    size_t ncalls[3] = {0, 0, 0};
     template <typename F>
     void timed(int cls, F&& f) {
-        if (!tim) {
+        if (!tim || (ncalls[cls]++ % 8) != 0) {
             f();
             return;
         }
         hipEvent_t a = ev(), b = ev();
         PF_HIP(hipEventRecord(a, S->stream));
-        f();
+        for (int r = 0; r < kTimingReps; ++r) f();
         PF_HIP(hipEventRecord(b, S->stream));
         evlog.push_back({cls, {a, b}});
     }
@@ -459,6 +466,7 @@ class Engine : public EngineBase {
         tim = on != 0;
         evlog.clear();
         evnext = 0;
+        ncalls[0] = ncalls[1] = ncalls[2] = 0;
     }
 
     double timing_ms(int which, int* count) override {
@@ -470,7 +478,7 @@ class Engine : public EngineBase {
                 float ms = 0.f;
                 PF_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
                 tot += ms;
-                ++c;
+                c += kTimingReps;
             }
         if (count) *count = c;
         return tot;
@@ -951,7 +959,10 @@ class BatchEngine : public BatchEngineBase {
     }
     size_t ncalls[3] = {0, 0, 0};
     // events bracket one launch in 8 of each kind (recording every launch would add a
-    // gap between dependent kernels); the sample still spans the whole timed region
+    // gap between dependent kernels); the sample still spans the whole timed region.  A
+    // sampled launch runs kTimingReps times back to back between its events, which spreads
+    // the events' own overhead; the repeats redo the launch on its own output, so the
+    // sweep's results are not meaningful in timing mode.
     std::vector<int> evlanes;        // lanes covered by each sampled launch
     template <typename F>
     void timed(int cls, hipStream_t s, int nl, F&& f) {
@@ -961,7 +972,7 @@ class BatchEngine : public BatchEngineBase {
         }
         hipEvent_t a = ev(), b = ev();
         PF_HIP(hipEventRecord(a, s));
-        f();
+        for (int r = 0; r < kTimingReps; ++r) f();
         PF_HIP(hipEventRecord(b, s));
         evlog.push_back({cls, {a, b}});
         evlanes.push_back(nl);
@@ -1122,7 +1133,7 @@ class BatchEngine : public BatchEngineBase {
                 float ms = 0.f;
                 PF_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
                 tot += ms;
-                ++c;
+                c += kTimingReps;
             }
         if (count) *count = c;
         return tot;
