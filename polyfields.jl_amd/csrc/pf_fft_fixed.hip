@@ -699,21 +699,25 @@ __device__ __forceinline__ float swap_pair(float x) {
 
 template <int NT>
 __global__ void __launch_bounds__(NT) k_pencil_pair(const Lane<float>* __restrict__ lanes, int ny,
-                                                    int per, int t) {
+                                                    int nxh, int t) {
     constexpr int NZ = 64, H = 32;
-    const int lane = blockIdx.x / per;
-    const int c = (blockIdx.x - lane * per) * (NT / 2) + (threadIdx.x >> 1);
+    // grid (column blocks, lanes); the lane fields come in one round of scalar loads, and
+    // a finished lane (t >= Ns) runs through with its stores going to an empty range
+    const Lane<float>* L = lanes + blockIdx.y;
+    unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf);
+    int nsteps = L->nsteps, s_in;
+    int b = __builtin_amdgcn_readfirstlane(step_block(L, t, &s_in));
+    asm volatile("" : "+s"(kraw), "+s"(nsteps), "+s"(b));
+    cpx<float>* const kbp = __builtin_bit_cast(cpx<float>*, kraw);
+    const int c = blockIdx.x * (NT / 2) + (threadIdx.x >> 1);
     const int half = threadIdx.x & 1;
-    const Lane<float>* L = lanes + lane;
-    if (t >= L->nsteps) return;
-    const int nxh = (L->nx >> 1) + 1, ncol = nxh * ny;
+    const int ncol = nxh * ny;
     const bool valid = c < ncol;
-    int s_in;
-    const int b = step_block(L, t, &s_in);
     // row 2m + half of column c: the half's row in the thread's offset, 2m rows in the
     // scalar one
     const unsigned row = (unsigned)ncol * 8u;
-    const SpecBuf<float> kb(L->kbuf, row * (unsigned)NZ);
+    const SpecBuf<float> kb(kbp, row * (unsigned)NZ);
+    const SpecBuf<float> kout(kbp, t < nsteps ? row * (unsigned)NZ : 0u);
     const unsigned off = (unsigned)(valid ? c : 0) * 8u + (unsigned)half * row;
     cpx<float> v[H], u[H];
 #pragma unroll
@@ -733,7 +737,7 @@ __global__ void __launch_bounds__(NT) k_pencil_pair(const Lane<float>* __restric
         const cpx<float> wo = cmul(twr<float, NZ, false>(k), o);
         v[k] = {e.x + sg * wo.x, e.y + sg * wo.y};
     }
-    const double mc = -L->ldc[b];
+    const double mc = -pick_blk(L->ldc, b);
     const double* G = L->G;
     const int ky = c / nxh, kx = c - ky * nxh;
     const double jx = (double)kx, jy = (double)fold(ky, ny);
@@ -758,7 +762,7 @@ __global__ void __launch_bounds__(NT) k_pencil_pair(const Lane<float>* __restric
     fft_reg<float, H, RegSplit<H>::R1, RegSplit<H>::R2, true>(v);
     if (valid) {
 #pragma unroll
-        for (int m = 0; m < H; ++m) kb.store(v[m], off, (unsigned)(2 * m) * row);
+        for (int m = 0; m < H; ++m) kout.store(v[m], off, (unsigned)(2 * m) * row);
     }
 }
 
@@ -878,8 +882,8 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
     if constexpr (sizeof(T) == 4) {
         if (pk == 3 && nz == 64) {
             const int ncol = (nx / 2 + 1) * ny, per = (ncol + 31) / 32;
-            hipLaunchKernelGGL((k_pencil_pair<64>), dim3(nl * per), dim3(64), 0, st,
-                               reinterpret_cast<const Lane<float>*>(d), ny, per, t);
+            hipLaunchKernelGGL((k_pencil_pair<64>), dim3(per, nl), dim3(64), 0, st,
+                               reinterpret_cast<const Lane<float>*>(d), ny, nx / 2 + 1, t);
             PF_HIP(hipGetLastError());
             return true;
         }
