@@ -20,6 +20,7 @@
  *   pf_system_set_flory ..... set_interaction!(FloryInteraction, a, b, chi)     interactions/flory.jl:25-37
  *   pf_system_set_edwards ... set_interaction!(EdwardsInteraction, a, u)        interactions/edwards.jl:25-32
  *   pf_system_set_field ..... sys.fields[mid] .= w (fieldinit! dict form)       fileio.jl:25-54
+ *   pf_system_fieldinit_seeded fieldinit!(sys; seed, scale)                     fileio.jl:14-23
  *   pf_system_get_grid ...... sys.fields / density / residuals / potentials     system/system.jl:35-38
  *   pf_system_get_species_density  species.density[mid]                         species/multiblock.jl:82
  *   pf_system_get_propagator  species.q[s] / species.qc[s]                      species/multiblock.jl:80-81
@@ -54,6 +55,8 @@
  */
 #ifndef POLYFIELDS_HIP_H
 #define POLYFIELDS_HIP_H
+
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -116,6 +119,17 @@ int pf_system_set_flory(pf_system* sys, int mid_a, int mid_b, double chi);
 int pf_system_set_edwards(pf_system* sys, int mid, double u);
 int pf_system_set_field(pf_system* sys, int mid, const double* src);
 int pf_system_get_grid(pf_system* sys, int which, int mid, double* dst);
+/* fieldinit!(sys; seed, scale) (fileio.jl:14-23): Random.seed!(seed), then
+   sys.fields[mid] .= scale * randn(dims) for each monomer in the iteration order of the
+   reference's Dict{Int} of fields (system/system.jl:35,122; monomers in the order they
+   were added), then residuals!.  The draws restate Julia 1.5's MersenneTwister (dSFMT-19937)
+   and randn (256-layer ziggurat), so the fields equal the reference's for the same seed. */
+int pf_system_fieldinit_seeded(pf_system* sys, uint64_t seed, double scale);
+/* The same stream without a system: Random.seed!(seed) then randn(counts[k]) for
+   k = 0..narrays-1, written one after another to out (sum of counts values).  Host code;
+   no GPU call.  No counterpart as a function in the reference: it exposes the draws of
+   fileio.jl:16-19 for checking. */
+int pf_julia_randn(uint64_t seed, const int64_t* counts, int narrays, double* out);
 /* Seeding from densities (fieldinit! with field_type = :rho, fileio.jl:25-54): write
    sys.density[mid], then pf_uniform_fields sets w = potentials(rho), mean-subtracted in
    the canonical ensemble (uniform_fields!, system/system.jl:348-357). */

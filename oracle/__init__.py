@@ -12,11 +12,9 @@ C-ordered numpy array of shape (Nz, Ny, Nx).  The half spectrum is taken along x
 reference's first dimension (``src/utils.jl:9``).
 
 Parity pins (what checks each function against something other than itself) are listed
-in ``tests/test_oracle_*.py``.  Functions with no independent pin:
-  * ``fieldinit`` draws from numpy's generator, not Julia 1.5's MersenneTwister, so the
-    reference's seeded known-answer values (test/test_system.jl:29-34,
-    test/test_updaters.jl:24,30) are not reproducible -- parity unpinned for those exact
-    trajectories; the seeded paths are pinned through invariants and converged states.
+in ``tests/test_oracle_*.py``.  ``fieldinit`` restates Julia 1.5's seeded draws
+(``julia_rng``), which the reference's own seeded known answers pin
+(test/test_system.jl:29-34, test/test_updaters.jl:24,30; tests/test_oracle_julia_rng.py).
 """
 
 from .cell import UnitCell, make_basis, basis_matrix, basis_dimensions  # noqa: F401

@@ -475,16 +475,19 @@ def _approx(a, b):
 
 
 def fieldinit(sys, seed=-1, scale=0.1, fields=None):
-    """src/fileio.jl:14-23: fields = scale * N(0,1), then ``residuals!``.
+    """src/fileio.jl:14-23: ``Random.seed!(seed)``, then ``scale * randn(dims)`` for each
+    monomer in the iteration order of the Dict of fields, then ``residuals!``.
 
-    The draws come from ``numpy.random.default_rng(seed)``, one ``standard_normal`` grid
-    per monomer in ascending id, not from Julia's MersenneTwister.  ``fields`` (a dict of
-    grids) replaces the random draw, as in the dict form at src/fileio.jl:25-54.
+    The draws restate Julia 1.5's MersenneTwister and randn (``julia_rng``), so the
+    fields equal the reference's for the same seed; a negative seed draws a fresh one.
+    ``fields`` (a dict of grids) replaces the random draw, as in the dict form at
+    src/fileio.jl:25-54.
     """
     if fields is None:
-        rng = np.random.default_rng(None if seed < 0 else seed)
-        for mid in sys.mids:
-            sys.fields[mid] = scale * rng.standard_normal(sys.shape)
+        from . import julia_rng
+        rng = julia_rng.seeded(seed if seed >= 0 else int(np.random.default_rng().integers(2 ** 62)))
+        for mid in julia_rng.dict_order(list(sys.monomers)):
+            sys.fields[mid] = scale * np.array(julia_rng.randn_array(rng, sys.ngrid)).reshape(sys.shape)
     else:
         for mid, w in fields.items():
             sys.fields[mid] = np.array(w, dtype=float).reshape(sys.shape)

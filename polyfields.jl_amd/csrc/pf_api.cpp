@@ -3,6 +3,7 @@
 
 #include "../../include/polyfields_hip.h"
 #include "pf_batch.h"
+#include "pf_julia_random.h"
 
 struct pf_system {
     pf::System* s;
@@ -184,6 +185,36 @@ int pf_system_set_field(pf_system* sys, int mid, const double* src) {
                               hipMemcpyHostToDevice, s.stream));
         s.sync();
         s.fields_set = true;
+    });
+}
+
+int pf_system_fieldinit_seeded(pf_system* sys, uint64_t seed, double scale) {
+    return guard([&] {
+        auto& s = S(sys);
+        pf::JuliaMT g(seed);
+        std::vector<double> buf(s.n);
+        for (int mid : pf::julia_dict_order(s.mon_order)) {
+            pf::julia_randn_array(g, buf.data(), (int64_t)s.n);
+            for (double& v : buf) v *= scale;
+            PF_HIP(hipMemcpyAsync(s.w + (size_t)s.slot_of(mid) * s.n, buf.data(),
+                                  sizeof(double) * s.n, hipMemcpyHostToDevice, s.stream));
+            s.sync();
+        }
+        s.fields_set = true;
+        s.residuals();
+    });
+}
+
+int pf_julia_randn(uint64_t seed, const int64_t* counts, int narrays, double* out) {
+    return guard([&] {
+        need(counts && out && narrays >= 0, "arguments");
+        pf::JuliaMT g(seed);
+        size_t off = 0;
+        for (int k = 0; k < narrays; ++k) {
+            need(counts[k] >= 0, "array length");
+            pf::julia_randn_array(g, out + off, counts[k]);
+            off += (size_t)counts[k];
+        }
     });
 }
 

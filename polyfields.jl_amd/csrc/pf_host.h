@@ -60,6 +60,7 @@ struct System {
     Cell cell;
 
     std::vector<int> mon_id;         // ascending monomer ids; slot = index
+    std::vector<int> mon_order;      // monomer ids in the order they were added
     std::vector<double> mon_vol, mon_charge;
     int has_flory = 0;
     double chi[PF_MAXMON * PF_MAXMON] = {0};

@@ -145,6 +145,7 @@ void System::add_monomer(int id, double vol, double charge) {
         return;
     }
     if (nmon() >= PF_MAXMON) throw Error(-2, "too many monomer types");
+    mon_order.push_back(id);
     size_t pos = 0;
     while (pos < mon_id.size() && mon_id[pos] < id) ++pos;
     mon_id.insert(mon_id.begin() + pos, id);

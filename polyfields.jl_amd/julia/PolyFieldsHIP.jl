@@ -134,7 +134,13 @@ function fieldinit!(sys::HIPSystem; seed::Integer = -1, scale::Real = 0.1)
     if seed >= 0
         Random.seed!(seed)
     end
-    for mid in sort(sys.mids)
+    # the draws follow the iteration order of the reference's Dict{Int} of fields, whose
+    # keys are inserted in monomer order (src/system/system.jl:35,122)
+    order = Dict{Int,Nothing}()
+    for mid in sys.mids
+        order[mid] = nothing
+    end
+    for mid in keys(order)
         setfield!(sys, mid, scale * randn(sys.dims))
     end
     residuals!(sys)
@@ -255,7 +261,9 @@ function fhelm_grad(sys::HIPSystem; functional::Symbol = :reference)
                  Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint),
                 sys.handle, PF_FHELM[functional], f, dw, dcell, np, dchi, du, dphi,
                 nb > 0 ? dN : C_NULL, nb))
-    dwd = Dict(mid => reshape(dw[(k - 1) * n + 1:k * n], sys.dims) for (k, mid) in enumerate(sys.mids))
+    # dw comes in slot order: ascending monomer ids
+    dwd = Dict(mid => reshape(dw[(k - 1) * n + 1:k * n], sys.dims)
+               for (k, mid) in enumerate(sort(sys.mids)))
     return f[], dwd, dcell, dchi, du, dphi, dN[1:nb]
 end
 
@@ -267,7 +275,7 @@ Vector-Jacobian product of `density!(sys)` (`pf_density_vjp`).
 function density_vjp(sys::HIPSystem, rho_bar::Dict, Q_bar::AbstractVector)
     nm = length(sys.mids)
     n = prod(sys.dims)
-    rb = reduce(vcat, (vec(Float64.(rho_bar[mid])) for mid in sys.mids))
+    rb = reduce(vcat, (vec(Float64.(rho_bar[mid])) for mid in sort(sys.mids)))
     dw = zeros(n * nm)
     dcell = zeros(length(sys.cell.params))
     nb = sum(sp isa Multiblock ? length(sp.N_block) : 0 for sp in sys.species; init = 0)
@@ -277,8 +285,8 @@ function density_vjp(sys::HIPSystem, rho_bar::Dict, Q_bar::AbstractVector)
                  Ptr{Float64}, Cint),
                 sys.handle, rb, Vector{Float64}(Q_bar), dw, dcell, length(dcell),
                 nb > 0 ? dN : C_NULL, nb))
-    return Dict(mid => reshape(dw[(k - 1) * n + 1:k * n], sys.dims) for (k, mid) in enumerate(sys.mids)),
-           dcell, dN[1:nb]
+    return Dict(mid => reshape(dw[(k - 1) * n + 1:k * n], sys.dims)
+                for (k, mid) in enumerate(sort(sys.mids))), dcell, dN[1:nb]
 end
 
 """

@@ -43,6 +43,8 @@ SIGNATURES = {
     "pf_system_set_flory": (I, [P, I, I, D]),
     "pf_system_set_edwards": (I, [P, I, D]),
     "pf_system_set_field": (I, [P, I, DP]),
+    "pf_system_fieldinit_seeded": (I, [P, C.c_uint64, D]),
+    "pf_julia_randn": (I, [C.c_uint64, C.POINTER(C.c_int64), I, DP]),
     "pf_system_get_grid": (I, [P, I, I, DP]),
     "pf_system_set_density": (I, [P, I, DP]),
     "pf_uniform_fields": (I, [P]),

@@ -13,6 +13,7 @@ operation is a call into the library.
 """
 import ctypes as C
 import math
+import os
 import warnings
 from dataclasses import dataclass
 
@@ -462,8 +463,10 @@ class FieldSystem:
 def fieldinit(sys, source=None, seed=-1, scale=0.1, fields=None, field_type="omega",
               load_cell=True, interpolate=True):
     """``fieldinit!`` (src/fileio.jl:14-64).  Forms: ``fieldinit(sys, seed=, scale=)`` --
-    scale*N(0,1) fields from numpy's ``default_rng(seed)``, one grid per monomer in
-    ascending id; ``fieldinit(sys, {mid: grid}, field_type=, interpolate=)``;
+    scale*N(0,1) fields drawn as the reference draws them (Julia 1.5's
+    ``Random.seed!(seed)`` and ``randn``, monomers in the reference's Dict order; a
+    negative seed draws a fresh one), so a seeded state equals the reference's;
+    ``fieldinit(sys, {mid: grid}, field_type=, interpolate=)``;
     ``fieldinit(sys, "file.json", load_cell=, interpolate=)``.  ``fields=`` is the dict
     form without interpolation.  Each form ends with ``residuals!``."""
     from . import fileio
@@ -472,12 +475,11 @@ def fieldinit(sys, source=None, seed=-1, scale=0.1, fields=None, field_type="ome
     if isinstance(source, dict):
         return fileio.fieldinit_dict(sys, source, field_type=field_type, interpolate=interpolate)
     if fields is None:
-        rng = np.random.default_rng(None if seed < 0 else seed)
-        for mid in sys.mids:
-            sys.fields[mid] = scale * rng.standard_normal(sys.shape)
-    else:
-        for mid, w in fields.items():
-            sys.fields[mid] = np.asarray(w, dtype=float).reshape(sys.shape)
+        s = seed if seed >= 0 else int.from_bytes(os.urandom(8), "little")
+        call("pf_system_fieldinit_seeded", sys._h, C.c_uint64(s), float(scale))
+        return
+    for mid, w in fields.items():
+        sys.fields[mid] = np.asarray(w, dtype=float).reshape(sys.shape)
     sys.residuals_update()
 
 

@@ -80,3 +80,23 @@ def test_errors_cross_the_abi_as_exceptions():
         P.UnitCell(2, "cubic", 4.0)
     with pytest.raises(ValueError):
         P.Diblock(P.Monomer(1), P.Monomer(2), 100, 0.29, 1.0, 1.0)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 12345, 2 ** 40 + 7, 2 ** 64 - 1])
+def test_julia_stream_matches_oracle(seed):
+    # pf_julia_randn (the library's host-side restatement of Julia 1.5's seeded randn,
+    # behind fieldinit!(sys; seed, scale)) against the oracle's, bit for bit, over array
+    # lengths on both sides of the 13-element bulk threshold and long enough to reach the
+    # ziggurat's slow paths and a second dSFMT block
+    import ctypes as C
+    from oracle import julia_rng as J
+    from polyfields_hip._lib import call
+    _lib()
+    counts = [1, 12, 13, 32, 1000, 5000, 7]
+    arr = (C.c_int64 * len(counts))(*counts)
+    out = np.zeros(sum(counts))
+    call("pf_julia_randn", C.c_uint64(seed), arr, len(counts),
+         out.ctypes.data_as(C.POINTER(C.c_double)))
+    g = J.seeded(seed)
+    ref = np.concatenate([np.array(J.randn_array(g, c)) for c in counts])
+    assert np.array_equal(out, ref)
