@@ -12,6 +12,59 @@
 
 namespace pf {
 
+HostPool::HostPool(int nworkers) {
+    for (int i = 0; i < nworkers; ++i) th.emplace_back([this, i] { worker(i); });
+}
+
+HostPool::~HostPool() {
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+    }
+    go.notify_all();
+    for (auto& t : th) t.join();
+}
+
+void HostPool::run(size_t n, const std::function<void(size_t, size_t)>& fn) {
+    const size_t parts = th.size() + 1, per = (n + parts - 1) / parts;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        job = &fn;
+        count = n;
+        pending = (int)th.size();
+        ++gen;
+    }
+    go.notify_all();
+    fn(0, std::min(n, per));
+    std::unique_lock<std::mutex> lk(mu);
+    done.wait(lk, [this] { return pending == 0; });
+    job = nullptr;
+}
+
+void HostPool::worker(int id) {
+    long seen = 0;
+    while (true) {
+        const std::function<void(size_t, size_t)>* fn;
+        size_t n;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            go.wait(lk, [&] { return stop || gen != seen; });
+            if (stop) return;
+            seen = gen;
+            fn = job;
+            n = count;
+        }
+        const size_t parts = th.size() + 1, per = (n + parts - 1) / parts;
+        const size_t b = std::min(n, (size_t)(id + 1) * per), e = std::min(n, (size_t)(id + 2) * per);
+        if (b < e) (*fn)(b, e);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            --pending;
+        }
+        done.notify_one();
+    }
+}
+
 Batch::Batch(const std::vector<System*>& s) : sys(s) {
     if (sys.empty()) throw Error(-1, "empty batch");
     precision = sys[0]->precision;
@@ -29,6 +82,11 @@ Batch::Batch(const std::vector<System*>& s) : sys(s) {
     d_step = (StepTask*)dev_alloc(sizeof(StepTask) * ns);
     PF_HIP(hipHostMalloc((void**)&h_step, sizeof(StepTask) * ns, hipHostMallocDefault));
     eng = make_batch_engine(*this);
+    if (ns >= 64) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int workers = (int)std::min(7u, hw > 1 ? hw - 1 : 0u);
+        if (workers > 0) pool = std::make_unique<HostPool>(workers);
+    }
 }
 
 Batch::~Batch() {
@@ -255,23 +313,30 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
             PF_HIP(hipMemcpyAsync(hsums, sums, sizeof(double) * 256 * ns, hipMemcpyDeviceToHost, stream));
             PF_HIP(hipStreamSynchronize(stream));
             const int nu = ih * (ih + 1) / 2;
-            std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
-            for (size_t k = 0; k < act.size(); ++k) {
-                const double* g = hsums + (size_t)act[k] * 256 + 64;
-                int q = 0;
-                for (int m = 0; m < ih; ++m)
-                    for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
-                for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
-                if (!lu_inverse(ih, U.data(), inv.data())) {
-                    // scft! raises on a singular Gram matrix (anderson.jl:203-208); in a
-                    // batch the system is reported as failed and dropped
-                    res[act[k]].diverged = 1;
-                    for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = 0.0;
-                    continue;
+            // each system's solve is independent: large batches split them over host threads
+            auto solve = [&](size_t k0, size_t k1) {
+                std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
+                for (size_t k = k0; k < k1; ++k) {
+                    const double* g = hsums + (size_t)act[k] * 256 + 64;
+                    int q = 0;
+                    for (int m = 0; m < ih; ++m)
+                        for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
+                    for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
+                    if (!lu_inverse(ih, U.data(), inv.data())) {
+                        // scft! raises on a singular Gram matrix (anderson.jl:203-208); in a
+                        // batch the system is reported as failed and dropped
+                        res[act[k]].diverged = 1;
+                        for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = 0.0;
+                        continue;
+                    }
+                    matvec(ih, inv.data(), V.data(), C.data());
+                    for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = C[nn];
                 }
-                matvec(ih, inv.data(), V.data(), C.data());
-                for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = C[nn];
-            }
+            };
+            if (pool && act.size() >= 64)
+                pool->run(act.size(), solve);
+            else
+                solve(0, act.size());
             PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
         }
         size_t Nmax = 0;

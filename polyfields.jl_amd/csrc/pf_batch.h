@@ -1,6 +1,10 @@
 // Batches of FieldSystems advanced together: every propagator launch covers the lanes
 // of all systems, and every system-level kernel covers all systems (blockIdx.y).
 #pragma once
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 
 #include <array>
 
@@ -85,9 +89,33 @@ struct BatchResult {
     int diverged;
 };
 
+// A few host threads that split a loop over [0, n) with the calling thread (the batch's
+// per-system Anderson solves).  Started with the batch, stopped with it.
+class HostPool {
+   public:
+    explicit HostPool(int nworkers);
+    ~HostPool();
+    // fn(begin, end) over consecutive chunks of [0, n), one per thread; returns when all
+    // are done
+    void run(size_t n, const std::function<void(size_t, size_t)>& fn);
+    int threads() const { return (int)th.size() + 1; }
+
+   private:
+    void worker(int id);
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable go, done;
+    const std::function<void(size_t, size_t)>* job = nullptr;
+    size_t count = 0;
+    long gen = 0;
+    int pending = 0;
+    bool stop = false;
+};
+
 class Batch {
    public:
     std::vector<System*> sys;
+    std::unique_ptr<HostPool> pool;  // Anderson solves of large batches
     hipStream_t stream = nullptr;
     int precision = 0;
     std::unique_ptr<BatchEngineBase> eng;

@@ -223,8 +223,29 @@ void chain_grid(const std::vector<double>& N_block, int Ns, double* N_out,
 // LU with partial pivoting (first maximal pivot), inverse by substitution against the
 // identity, then a row-by-row product: the same arithmetic order as the CPU oracle.
 bool lu_inverse(int n, const double* U, double* inv) {
-    std::vector<double> A(U, U + n * n);
-    std::vector<int> perm(n);
+    // LU with partial pivoting, then the inverse column by column (inv!(lu!(U)),
+    // updaters/anderson.jl:185).  The substitutions run row by row over all n right-hand
+    // sides at once: every element sees the same operations in the same order as one
+    // column at a time, and the independent columns vectorise.  Working storage is on the
+    // stack for the sizes used here (Anderson histories up to PF_MAXHIST, cell Jacobians),
+    // on the heap beyond.
+    constexpr int kSmall = 16;
+    double a_s[kSmall * kSmall], x_s[kSmall * kSmall];
+    int perm_s[kSmall];
+    std::vector<double> a_h, x_h;
+    std::vector<int> perm_h;
+    double* A = a_s;
+    double* X = x_s;
+    int* perm = perm_s;
+    if (n > kSmall) {
+        a_h.resize((size_t)n * n);
+        x_h.resize((size_t)n * n);
+        perm_h.resize(n);
+        A = a_h.data();
+        X = x_h.data();
+        perm = perm_h.data();
+    }
+    for (int i = 0; i < n * n; ++i) A[i] = U[i];
     for (int i = 0; i < n; ++i) perm[i] = i;
     for (int k = 0; k < n; ++k) {
         int p = k;
@@ -245,20 +266,28 @@ bool lu_inverse(int n, const double* U, double* inv) {
             for (int j = k + 1; j < n; ++j) A[i * n + j] = A[i * n + j] - f * A[k * n + j];
         }
     }
-    std::vector<double> y(n), x(n);
-    for (int c = 0; c < n; ++c) {
-        for (int i = 0; i < n; ++i) {
-            double acc = (perm[i] == c) ? 1.0 : 0.0;
-            for (int j = 0; j < i; ++j) acc -= A[i * n + j] * y[j];
-            y[i] = acc;
+    // forward substitution: row i of L^-1 P (X holds Y, row-major [i][c])
+    for (int i = 0; i < n; ++i) {
+        double* xi = X + (size_t)i * n;
+        for (int c = 0; c < n; ++c) xi[c] = (perm[i] == c) ? 1.0 : 0.0;
+        for (int j = 0; j < i; ++j) {
+            const double f = A[i * n + j];
+            const double* xj = X + (size_t)j * n;
+            for (int c = 0; c < n; ++c) xi[c] -= f * xj[c];
         }
-        for (int i = n - 1; i >= 0; --i) {
-            double acc = y[i];
-            for (int j = i + 1; j < n; ++j) acc -= A[i * n + j] * x[j];
-            x[i] = acc / A[i * n + i];
-        }
-        for (int i = 0; i < n; ++i) inv[i * n + c] = x[i];
     }
+    // back substitution in place: rows from the last up
+    for (int i = n - 1; i >= 0; --i) {
+        double* xi = X + (size_t)i * n;
+        for (int j = i + 1; j < n; ++j) {
+            const double f = A[i * n + j];
+            const double* xj = X + (size_t)j * n;
+            for (int c = 0; c < n; ++c) xi[c] -= f * xj[c];
+        }
+        const double d = A[i * n + i];
+        for (int c = 0; c < n; ++c) xi[c] = xi[c] / d;
+    }
+    for (int i = 0; i < n * n; ++i) inv[i] = X[i];
     return true;
 }
 

@@ -459,3 +459,26 @@ def test_fieldinit_from_densities_fp64():
     for m in (1, 2):
         assert rel(sg.fields[m], so.fields[m]) < 1e-12
         assert rel(sg.density[m], so.density[m]) < 1e-12
+
+
+def test_large_batch_matches_one_at_a_time_fp64():
+    # 64 systems: the per-system Anderson solves run on several host threads; every
+    # system's trajectory must still equal its own one-at-a-time run
+    # chi N below the order-disorder transition: the random seeds relax without diverging
+    cfgs = [(16, 4.0 + 0.02 * k, 30 + (k % 15), 6.0 + 0.05 * k, 40) for k in range(64)]
+
+    def make(k):
+        s = _hex_system(P, *cfgs[k], precision="fp64")
+        P.fieldinit(s, seed=500 + k, scale=0.1)
+        return s
+
+    batch_sys = [make(k) for k in range(64)]
+    res = P.Batch(batch_sys).scft(nhist=10, iter_euler=8, lam_euler=0.05, nsteps=25,
+                                  ftol=1e-14)
+    for k in (0, 17, 42, 63):
+        q = P.scft(make(k), P.AndersonUpdater(nhist=10, iter_euler=8, lam_euler=0.05),
+                   opts=P.SCFTOptions(nsteps=25, ftol=1e-14))
+        assert np.isfinite(q.fhelm)
+        assert res[k].nsteps == q.nsteps
+        assert res[k].fhelm == pytest.approx(q.fhelm, rel=1e-13)
+        assert res[k].ferr == pytest.approx(q.ferr, rel=1e-12)
