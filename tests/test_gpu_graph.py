@@ -75,3 +75,28 @@ def test_graph_batched_scft_equals_eager():
         r = batch.scft(nhist=10, iter_euler=1000000000, lam_euler=0.1, nsteps=60, ftol=1e-12)
         res[mode] = [(x.nsteps, x.ferr, x.fhelm) for x in r]
     assert res[True] == res[False]
+
+
+def test_graph_sweep_of_64_systems_equals_eager_fp32():
+    # 64 two-dimensional systems on four grid sizes (four lane groups on their own
+    # streams inside the graph), 40 replayed FP32 Euler iterations: every system's free
+    # energy equals the eager run's bit for bit, in repeated runs
+    def run(graph):
+        ss = []
+        for k in range(64):
+            n = (32, 48, 64, 96)[k % 4]
+            s, _, _ = diblock_system(P, (n, n, 1), 2, "hexagonal", (4.0 + 0.03 * k,), N=100,
+                                     f=(0.3, 0.35, 0.4)[k % 3], chi=0.12 + 0.001 * k, Ns=40 + (k % 9),
+                                     precision="fp32")
+            rng = np.random.default_rng(700 + k)
+            for m in (1, 2):
+                s.fields[m] = 0.1 * rng.standard_normal(s.shape)
+            ss.append(s)
+        b = P.Batch(ss)
+        b.use_graph(graph)
+        r = b.scft(nhist=10, iter_euler=1000, lam_euler=0.1, nsteps=40, ftol=1e-14)
+        return np.array([x.fhelm for x in r])
+
+    ref = run(False)
+    for _ in range(3):
+        np.testing.assert_array_equal(run(True), ref)
