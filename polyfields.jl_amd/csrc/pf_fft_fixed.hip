@@ -20,9 +20,23 @@ namespace pf {
 template <typename T, int NX, int NY, int NT>
 __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ lanes, int t0, int t1) {
     constexpr int M = NX / 2, LD = M + 1, N = NX * NY, NKP = NY * LD;
+    // The LD factors exp(-c_b k^2) of the current block are computed when the block
+    // changes, not in every step, where the plane, the twiddles and the factors fit in
+    // 160 KiB of LDS.  In the 1024-thread forms (one workgroup fills a CU's wave slots,
+    // so LDS does not limit occupancy), the block's LW pairs are copied there at the same
+    // time when they fit too, instead of being read from global memory twice per step.
+    // Entry i of either table is written and read only by thread i % NT, so they need no
+    // barrier.
+    constexpr int NP = M * NY;
+    constexpr size_t LDS_BASE = (size_t)NKP * sizeof(cpx<T>) + (size_t)(NX + NY) * sizeof(cpx<T>);
+    constexpr bool FTAB = LDS_BASE + (size_t)NKP * sizeof(T) <= 160u * 1024u;
+    constexpr bool LTAB = FTAB && NT == 1024 &&
+                          LDS_BASE + (size_t)NKP * sizeof(T) + (size_t)NP * sizeof(cpx<T>) <= 160u * 1024u;
     __shared__ cpx<T> S[NKP];
     __shared__ cpx<T> TWX[NX];
     __shared__ cpx<T> TWY[NY];
+    __shared__ T FL[FTAB ? NKP : 1];
+    __shared__ cpx<T> LWT[LTAB ? NP : 1];
     const Lane<T>* L = lanes + blockIdx.x;
     const int tend = min(t1, L->nsteps);
     if (t0 >= tend) return;
@@ -36,33 +50,63 @@ __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ la
         for_pairs<T, NX, NY, NT>([&](int i, int o) { S[o] = qin[i]; });
     }
     const double* G = L->G;
+    int bf = -1;   // block whose tables FL (and LWT) hold
     for (int t = t0; t < tend; ++t) {
         const int b = step_block(L, t, &s_in);
         const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b]));
+        if constexpr (FTAB) {
+            if (b != bf) {
+                bf = b;
+                if constexpr (LTAB) for_pairs<T, NX, NY, NT>([&](int i, int) { LWT[i] = lw[i]; });
+                const double mc = -L->ldc[b];
+                // not unrolled: the fill is rare, and one exp at a time keeps the
+                // step loop's register budget
+#pragma unroll 1
+                for (int i0 = 0; i0 < NKP; i0 += NT) {
+                    const int i = i0 + threadIdx.x;
+                    if (i < NKP) {
+                        const int y = i / LD, kx = i % LD;
+                        FL[i] = exp_t<T>(mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0));
+                    }
+                }
+            }
+        }
         __syncthreads();
         for_pairs<T, NX, NY, NT>([&](int i, int o) {
-            const cpx<T> w = lw[i];
+            const cpx<T> w = LTAB ? LWT[i] : lw[i];
             S[o].x *= w.x;
             S[o].y *= w.y;
         });
         __syncthreads();
         cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
-        const double mc = -L->ldc[b];
+        if constexpr (FTAB) {
 #pragma unroll
-        for (int i0 = 0; i0 < NKP; i0 += NT) {
-            const int i = i0 + threadIdx.x;
-            if (i < NKP) {
-                const int y = i / LD, kx = i % LD;
-                const T f = exp_t<T>(mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0));
-                S[i].x *= f;
-                S[i].y *= f;
+            for (int i0 = 0; i0 < NKP; i0 += NT) {
+                const int i = i0 + threadIdx.x;
+                if (i < NKP) {
+                    const T f = FL[i];
+                    S[i].x *= f;
+                    S[i].y *= f;
+                }
+            }
+        } else {
+            const double mc = -L->ldc[b];
+#pragma unroll
+            for (int i0 = 0; i0 < NKP; i0 += NT) {
+                const int i = i0 + threadIdx.x;
+                if (i < NKP) {
+                    const int y = i / LD, kx = i % LD;
+                    const T f = exp_t<T>(mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0));
+                    S[i].x *= f;
+                    S[i].y *= f;
+                }
             }
         }
         __syncthreads();
         cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
         const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * N));
         for_pairs<T, NX, NY, NT>([&](int i, int o) {
-            const cpx<T> w = lw[i];
+            const cpx<T> w = LTAB ? LWT[i] : lw[i];
             const cpx<T> v = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
             S[o] = v;
             qout[i] = v;
