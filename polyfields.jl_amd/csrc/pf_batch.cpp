@@ -76,9 +76,11 @@ Batch::Batch(const std::vector<System*>& s) : sys(s) {
     }
     PF_HIP(hipSetDevice(sys[0]->device));
     PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    PF_HIP(hipEventCreateWithFlags(&sums_ev, hipEventDisableTiming));
     const size_t ns = sys.size();
     sums = (double*)dev_alloc(sizeof(double) * 256 * ns);
-    PF_HIP(hipHostMalloc((void**)&hsums, sizeof(double) * 256 * ns, hipHostMallocDefault));
+    csums = (double*)dev_alloc(sizeof(double) * kCompact * ns);
+    PF_HIP(hipHostMalloc((void**)&hcsums, sizeof(double) * kCompact * ns, hipHostMallocDefault));
     d_step = (StepTask*)dev_alloc(sizeof(StepTask) * ns);
     PF_HIP(hipHostMalloc((void**)&h_step, sizeof(StepTask) * ns, hipHostMallocDefault));
     eng = make_batch_engine(*this);
@@ -95,10 +97,12 @@ Batch::~Batch() {
     drop_graphs();
     eng.reset();
     dev_free(sums);
+    dev_free(csums);
     dev_free(d_step);
     dev_free(d_act);
-    if (hsums) (void)hipHostFree(hsums);
+    if (hcsums) (void)hipHostFree(hcsums);
     if (h_step) (void)hipHostFree(h_step);
+    if (sums_ev) (void)hipEventDestroy(sums_ev);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -148,6 +152,8 @@ void Batch::bind_active(const std::vector<int>& act) {
         k.sums = sums + (size_t)i * 256;
         t.push_back(k);
     }
+    // work queued earlier may still read the old descriptors
+    PF_HIP(hipStreamSynchronize(stream));
     dev_free(d_act);
     d_act = (SysTask*)dev_alloc(sizeof(SysTask) * t.size());
     PF_HIP(hipMemcpyAsync(d_act, t.data(), sizeof(SysTask) * t.size(), hipMemcpyHostToDevice, stream));
@@ -210,14 +216,23 @@ void Batch::density(const std::vector<int>& act) { run(0, act); }
 
 void Batch::residuals(const std::vector<int>& act) { run(1, act); }
 
-void Batch::sys_sums(const std::vector<int>& act, std::vector<double>& out) {
+void Batch::sums_launch(const std::vector<int>& act) {
     bind_active(act);
-    launch_sys_reduce_b(d_act, (int)act.size(), max_parts(tasks, false), stream);
-    PF_HIP(hipMemcpyAsync(hsums, sums, sizeof(double) * 256 * sys.size(), hipMemcpyDeviceToHost, stream));
-    PF_HIP(hipStreamSynchronize(stream));
+    launch_sys_reduce_b(d_act, (int)act.size(), max_parts(tasks, false), csums, stream);
+    PF_HIP(hipMemcpyAsync(hcsums, csums, sizeof(double) * 5 * act.size(), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipEventRecord(sums_ev, stream));
+}
+
+void Batch::sums_read(const std::vector<int>& act, std::vector<double>& out) {
+    PF_HIP(hipEventSynchronize(sums_ev));
     out.assign(sys.size() * 5, 0.0);
-    for (int i : act)
-        for (int k = 0; k < 5; ++k) out[(size_t)i * 5 + k] = hsums[(size_t)i * 256 + 32 + k];
+    for (size_t j = 0; j < act.size(); ++j)
+        for (int k = 0; k < 5; ++k) out[(size_t)act[j] * 5 + k] = hcsums[j * 5 + k];
+}
+
+void Batch::sys_sums(const std::vector<int>& act, std::vector<double>& out) {
+    sums_launch(act);
+    sums_read(act, out);
 }
 
 static double ferr_of(const double* v) {
@@ -271,6 +286,10 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
     }
     act = next;
     int step = 0;
+    bool queued = false;   // the residuals of the current active set are already queued
+    // fresh[i]: system i's last device work is a residuals pass queued after its last mix,
+    // which is then its final state (scft! ends with one residuals! after the last step)
+    std::vector<char> fresh(ns, 0);
     while (!act.empty() && step < nsteps) {
         // bookkeeping shared by the lockstep systems
         Anderson& A0 = *and_state[act[0]];
@@ -289,7 +308,11 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
             and_state[i]->iter_hist = ih;
             and_state[i]->lam = lam;
         }
-        residuals(act);
+        if (queued)
+            bind_active(act);
+        else
+            residuals(act);
+        queued = false;
         const bool euler = iter <= and_state[act[0]]->iter_euler;
         for (size_t k = 0; k < act.size(); ++k) {
             Anderson& A = *and_state[act[k]];
@@ -309,15 +332,15 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
         }
         PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
         if (!euler) {
-            launch_gram_b(ih, d_act, d_step, (int)act.size(), max_parts(tasks, true), stream);
-            PF_HIP(hipMemcpyAsync(hsums, sums, sizeof(double) * 256 * ns, hipMemcpyDeviceToHost, stream));
+            const int nu = ih * (ih + 1) / 2, nv = nu + ih;
+            launch_gram_b(ih, d_act, d_step, (int)act.size(), max_parts(tasks, true), csums, stream);
+            PF_HIP(hipMemcpyAsync(hcsums, csums, sizeof(double) * nv * act.size(), hipMemcpyDeviceToHost, stream));
             PF_HIP(hipStreamSynchronize(stream));
-            const int nu = ih * (ih + 1) / 2;
             // each system's solve is independent: large batches split them over host threads
             auto solve = [&](size_t k0, size_t k1) {
                 std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
                 for (size_t k = k0; k < k1; ++k) {
-                    const double* g = hsums + (size_t)act[k] * 256 + 64;
+                    const double* g = hcsums + k * (size_t)nv;
                     int q = 0;
                     for (int m = 0; m < ih; ++m)
                         for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
@@ -342,9 +365,21 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
         size_t Nmax = 0;
         for (int i : act) Nmax = std::max(Nmax, sys[i]->n * sys[i]->nmon());
         launch_mix_b(d_act, d_step, (int)act.size(), Nmax, stream);
+        for (int i : act) fresh[i] = 0;
         if (iter >= nhist + 1)
             for (int i : act) and_state[i]->base = (and_state[i]->base + 1) % (nhist + 1);
-        sys_sums(act, v);
+        sums_launch(act);
+        // The next iteration's residuals (density! of the mixed fields) are queued before
+        // the convergence test reads its sums, so the device does not wait for the host
+        // between iterations.  Each system's residuals depend on its own fields only: for
+        // the systems that stay active they are what the next iteration computes, and for
+        // a system that the test drops they are the final residuals! of scft!.
+        if (step + 1 < nsteps) {
+            residuals(act);
+            queued = true;
+            for (int i : act) fresh[i] = 1;
+        }
+        sums_read(act, v);
         next.clear();
         for (int i : act) {
             const double fe = ferr_of(&v[(size_t)i * 5]);
@@ -363,13 +398,20 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
                 next.push_back(i);
             }
         }
+        // a smaller active set gets new descriptors and lane tables: the queued pass must
+        // have read the old ones before they are freed
+        if (queued && next != act) PF_HIP(hipStreamSynchronize(stream));
         act = next;
         step += 1;
     }
     for (int i : act) res[i].nsteps = step;
-    // final state (src/simulate.jl:133-136)
+    // final state (src/simulate.jl:133-136): one residuals! after each system's last step
+    // (already queued for the systems dropped with a pass in flight)
     const std::vector<int>& all = run;
-    residuals(all);
+    std::vector<int> need;
+    for (int i : all)
+        if (!fresh[i]) need.push_back(i);
+    if (!need.empty()) residuals(need);
     sys_sums(all, v);
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (int i : all) {

@@ -56,9 +56,9 @@ void launch_lw_fill_b(const LwTask<T>* lw, int nlw, const FillTask<T>* fill, int
                       size_t nmax, hipStream_t st);
 void launch_muphi_pool_b(const SysTask* d, int nsys, size_t nmax, hipStream_t st);
 void launch_residuals_b(const SysTask* d, int nsys, int maxparts, size_t nmax, hipStream_t st);
-void launch_sys_reduce_b(const SysTask* d, int nsys, int maxparts, hipStream_t st);
+void launch_sys_reduce_b(const SysTask* d, int nsys, int maxparts, double* compact, hipStream_t st);
 void launch_gram_b(int h, const SysTask* d, const StepTask* s, int nsys, int maxparts,
-                   hipStream_t st);
+                   double* compact, hipStream_t st);
 void launch_mix_b(const SysTask* d, const StepTask* s, int nsys, size_t Nmax, hipStream_t st);
 
 class BatchEngineBase {
@@ -120,7 +120,11 @@ class Batch {
     int precision = 0;
     std::unique_ptr<BatchEngineBase> eng;
     double* sums = nullptr;          // nsys * 256 device doubles
-    double* hsums = nullptr;         // pinned mirror
+    // per active system, in active-set order, the values the host reads (the five system
+    // sums, or the Gram entries): device copy and pinned mirror, nsys * kCompact doubles
+    static constexpr int kCompact = PF_MAXHIST * (PF_MAXHIST + 3) / 2;
+    double* csums = nullptr;
+    double* hcsums = nullptr;
     SysTask* d_sys = nullptr;        // all systems' descriptors
     StepTask* d_step = nullptr;
     StepTask* h_step = nullptr;      // pinned staging
@@ -147,6 +151,11 @@ class Batch {
     void density(const std::vector<int>& act);
     void residuals(const std::vector<int>& act);
     void sys_sums(const std::vector<int>& act, std::vector<double>& out);  // 5 per system
+    // sys_sums in two halves: queue the reduction and its copy to the host, then wait for
+    // that copy only (work queued in between keeps running)
+    void sums_launch(const std::vector<int>& act);
+    void sums_read(const std::vector<int>& act, std::vector<double>& out);
+    hipEvent_t sums_ev = nullptr;
     std::vector<BatchResult> scft_anderson(int nhist, int iter_euler, double lam_euler,
                                            int nsteps, double ftol, const int* mask);
 };

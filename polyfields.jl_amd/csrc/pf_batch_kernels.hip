@@ -128,8 +128,11 @@ __global__ void __launch_bounds__(RB) k_residuals_b(const SysTask* __restrict__ 
     wg_store<2 * PF_MAXMON>(v, t->partial + (size_t)blockIdx.x * 2 * PF_MAXMON);
 }
 
-// per system: sums[off + k] = sum_p partial[p*nv + k]  (p in order)
-__global__ void k_final_b(const SysTask* __restrict__ tasks, int nv, int off, int use_gram) {
+// per system: sums[off + k] = sum_p partial[p*nv + k]  (p in order); with `compact`, the
+// same values also go to compact[j*nv + k] for the j-th active system, so that the host
+// copies nv values per active system rather than every system's sums row
+__global__ void k_final_b(const SysTask* __restrict__ tasks, int nv, int off, int use_gram,
+                          double* compact) {
     const SysTask* t = tasks + blockIdx.x;
     const double* part = use_gram ? t->gpartial : t->partial;
     const int np = use_gram ? t->gparts : t->nparts;
@@ -137,6 +140,7 @@ __global__ void k_final_b(const SysTask* __restrict__ tasks, int nv, int off, in
         double s = 0.0;
         for (int p = 0; p < np; ++p) s += part[(size_t)p * nv + k];
         t->sums[off + k] = s;
+        if (compact) compact[(size_t)blockIdx.x * nv + k] = s;
     }
 }
 
@@ -157,7 +161,7 @@ __global__ void k_mean_subtract_b(const SysTask* __restrict__ tasks) {
 
 void launch_residuals_b(const SysTask* d, int nsys, int maxparts, size_t nmax, hipStream_t st) {
     hipLaunchKernelGGL(k_residuals_b, dim3(maxparts, nsys), dim3(RB), 0, st, d);
-    hipLaunchKernelGGL(k_final_b, dim3(nsys), dim3(64), 0, st, d, 2 * PF_MAXMON, 0, 0);
+    hipLaunchKernelGGL(k_final_b, dim3(nsys), dim3(64), 0, st, d, 2 * PF_MAXMON, 0, 0, nullptr);
     const int g = (int)std::min<size_t>((nmax + 255) / 256, 1024);
     hipLaunchKernelGGL(k_mean_subtract_b, dim3(g, nsys), dim3(256), 0, st, d);
     PF_HIP(hipGetLastError());
@@ -187,9 +191,9 @@ __global__ void __launch_bounds__(RB) k_sys_reduce_b(const SysTask* __restrict__
     wg_store<5>(v, t->partial + (size_t)blockIdx.x * 5);
 }
 
-void launch_sys_reduce_b(const SysTask* d, int nsys, int maxparts, hipStream_t st) {
+void launch_sys_reduce_b(const SysTask* d, int nsys, int maxparts, double* compact, hipStream_t st) {
     hipLaunchKernelGGL(k_sys_reduce_b, dim3(maxparts, nsys), dim3(RB), 0, st, d);
-    hipLaunchKernelGGL(k_final_b, dim3(nsys), dim3(64), 0, st, d, 5, 32, 0);
+    hipLaunchKernelGGL(k_final_b, dim3(nsys), dim3(64), 0, st, d, 5, 32, 0, compact);
     PF_HIP(hipGetLastError());
 }
 
@@ -222,15 +226,16 @@ __global__ void __launch_bounds__(RB) k_gram_b(const SysTask* __restrict__ tasks
 }
 
 template <int H>
-static void gram_b(const SysTask* d, const StepTask* s, int nsys, int maxparts, hipStream_t st) {
+static void gram_b(const SysTask* d, const StepTask* s, int nsys, int maxparts, double* compact,
+                   hipStream_t st) {
     hipLaunchKernelGGL(k_gram_b<H>, dim3(maxparts, nsys), dim3(RB), 0, st, d, s);
-    hipLaunchKernelGGL(k_final_b, dim3(nsys), dim3(64), 0, st, d, H * (H + 1) / 2 + H, 64, 1);
+    hipLaunchKernelGGL(k_final_b, dim3(nsys), dim3(64), 0, st, d, H * (H + 1) / 2 + H, 64, 1, compact);
 }
 
 void launch_gram_b(int h, const SysTask* d, const StepTask* s, int nsys, int maxparts,
-                   hipStream_t st) {
+                   double* compact, hipStream_t st) {
     switch (h) {
-#define PF_G(H) case H: gram_b<H>(d, s, nsys, maxparts, st); break;
+#define PF_G(H) case H: gram_b<H>(d, s, nsys, maxparts, compact, st); break;
         PF_G(1) PF_G(2) PF_G(3) PF_G(4) PF_G(5) PF_G(6) PF_G(7) PF_G(8)
         PF_G(9) PF_G(10) PF_G(11) PF_G(12) PF_G(13) PF_G(14) PF_G(15) PF_G(16)
 #undef PF_G

@@ -1036,11 +1036,19 @@ class BatchEngine : public BatchEngineBase {
                 if (e.qc_stack((int)k)) fillt.push_back({e.qc_stack((int)k) + (size_t)sp.Ns * s.n, s.n});
             }
         }
+        // The groups are launched largest first (grid points x contour steps): they run
+        // concurrently, and the longest one, which the batch waits for, then starts first.
+        std::vector<size_t> order(g.size());
+        for (size_t k = 0; k < g.size(); ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return (double)g[a].nx * g[a].ny * g[a].nz * g[a].maxNs >
+                   (double)g[b].nx * g[b].ny * g[b].nz * g[b].maxNs;
+        });
         // 3-D groups are split into `split` sub-groups on streams of their own, so that
         // the slab launches of one sub-group overlap the pencil launches of another
         lanes.clear();
         std::vector<Group> fin;
-        for (size_t k = 0; k < g.size(); ++k) {
+        for (size_t k : order) {
             const int cnt = (int)by_size[k].size();
             const int sp = schedule(sizeof(T), g[k].nx, g[k].ny, g[k].nz).split;
             const int want = sp > 0 ? sp : split;
