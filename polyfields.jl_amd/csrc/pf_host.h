@@ -114,6 +114,9 @@ struct System {
     void sys_sums(double out[5]);    // [res^2, w^2, rho*w, flory, edwards]
     double field_error();
     double free_energy();
+    // both from one reduction (what scft! reads after every step)
+    void errors(double* ferr, double* fhelm);
+    double free_energy_of(const double v[5]);
     double free_energy_bulk();
     std::vector<double> scfstress(int exact = 0);
     void monomer_fractions(std::vector<double>& fr);

@@ -404,9 +404,20 @@ double System::field_error() {
 }
 
 double System::free_energy() {
-    // src/system/system.jl:402-433 (mu = phi, as in the reference)
     double v[5];
     sys_sums(v);
+    return free_energy_of(v);
+}
+
+void System::errors(double* ferr, double* fhelm) {
+    double v[5];
+    sys_sums(v);
+    *ferr = std::sqrt(v[0]) / std::sqrt(EPS + v[1]);
+    *fhelm = free_energy_of(v);
+}
+
+double System::free_energy_of(const double v[5]) {
+    // src/system/system.jl:402-433 (mu = phi, as in the reference)
     if (!canonical) fetch_scalars();
     double ftrans = 0.0;
     for (size_t i = 0; i < species.size(); ++i) {
@@ -708,8 +719,8 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
     s.residuals();
     u.setup(&s);
     int step = 0;
-    double ferr = s.field_error();
-    double fhelm = s.free_energy();
+    double ferr, fhelm;
+    s.errors(&ferr, &fhelm);
     double serr = 0.0;
     const bool vcell = d != nullptr;
     if (vcell) {
@@ -730,16 +741,14 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
                 if (o.nsout > 0) std::printf("Updating cell dimensions: serr = %.5e\n", serr);
             }
         }
-        ferr = s.field_error();
-        fhelm = s.free_energy();
+        s.errors(&ferr, &fhelm);
         if (std::isnan(ferr)) throw Error(-5, "Diverging field updater -- breaking.");
         conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
         if (conv) break;
         step += 1;
     }
     s.residuals();
-    ferr = s.field_error();
-    fhelm = s.free_energy();
+    s.errors(&ferr, &fhelm);
     if (vcell) serr = d->stress_error();
     ScftResults r;
     r.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
