@@ -83,6 +83,10 @@ struct System {
     double* sums = nullptr;          // small result area (64 doubles)
     double* pinned = nullptr;        // pinned host staging (4096 doubles)
     hipStream_t stream = nullptr;
+    hipEvent_t err_ev = nullptr;     // marks the copy of the five system sums to the host
+    // residuals! of the current fields is already queued (scft! queues the next step's
+    // before it reads the errors); the next residuals() call only clears the flag
+    bool res_queued = false;
 
     std::unique_ptr<EngineBase> eng;
     bool fields_set = false;
@@ -114,8 +118,11 @@ struct System {
     void sys_sums(double out[5]);    // [res^2, w^2, rho*w, flory, edwards]
     double field_error();
     double free_energy();
-    // both from one reduction (what scft! reads after every step)
+    // both from one reduction (what scft! reads after every step); errors_launch queues
+    // the reduction and its copy, errors_read waits for that copy only
     void errors(double* ferr, double* fhelm);
+    void errors_launch();
+    void errors_read(double* ferr, double* fhelm);
     double free_energy_of(const double v[5]);
     double free_energy_bulk();
     std::vector<double> scfstress(int exact = 0);
@@ -178,6 +185,9 @@ struct Updater {
     virtual ~Updater() = default;
     virtual void setup(System* s) = 0;
     virtual void step() = 0;
+    // true when step() starts with residuals! of the fields as they are (nothing changes
+    // them before), so that scft! may queue that pass early
+    virtual bool starts_with_residuals() const { return false; }
     System* sys = nullptr;
 };
 
@@ -197,6 +207,7 @@ struct Anderson : Updater {
     double* wslot(int logical) const;
     void setup(System* s) override;
     void step() override;
+    bool starts_with_residuals() const override { return true; }
 };
 
 struct Euler : Updater {

@@ -92,6 +92,7 @@ System::~System() {
     dev_free(g_part);
     dev_free(g_out);
     if (pinned) (void)hipHostFree(pinned);
+    if (err_ev) (void)hipEventDestroy(err_ev);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -364,6 +365,10 @@ void System::density() {
 
 void System::residuals() {
     // src/system/system.jl:317-340
+    if (res_queued) {
+        res_queued = false;
+        return;
+    }
     density();
     launch_residuals(residual_args(), n, partial, reduce_parts(n), sums, stream);
 }
@@ -410,8 +415,21 @@ double System::free_energy() {
 }
 
 void System::errors(double* ferr, double* fhelm) {
+    errors_launch();
+    errors_read(ferr, fhelm);
+}
+
+void System::errors_launch() {
+    if (!err_ev) PF_HIP(hipEventCreateWithFlags(&err_ev, hipEventDisableTiming));
+    launch_sys_reduce(residual_args(), n, partial, reduce_parts(n), sums + 32, stream);
+    PF_HIP(hipMemcpyAsync(pinned, sums + 32, sizeof(double) * 5, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipEventRecord(err_ev, stream));
+}
+
+void System::errors_read(double* ferr, double* fhelm) {
+    PF_HIP(hipEventSynchronize(err_ev));
     double v[5];
-    sys_sums(v);
+    for (int k = 0; k < 5; ++k) v[k] = pinned[k];
     *ferr = std::sqrt(v[0]) / std::sqrt(EPS + v[1]);
     *fhelm = free_energy_of(v);
 }
@@ -728,6 +746,16 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
         serr = d->stress_error();
     }
     bool conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
+    // The next step's residuals! are queued before the host reads this step's errors
+    // when nothing can change the fields or the cell in between: an updater whose step
+    // starts with residuals!, a fixed cell, and the canonical ensemble (the grand free
+    // energy reads device scalars that the queued pass would overwrite).  A converged
+    // system then takes the queued pass as its final residuals!.
+    const bool queue = u.starts_with_residuals() && !vcell && s.canonical;
+    struct Unqueue {
+        System& s;
+        ~Unqueue() { s.res_queued = false; }
+    } unqueue{s};
     while (step < o.nsteps && !conv) {
         if (o.nsout > 0 && step % o.nsout == 0)
             std::printf("Step (%-5d / %5d): ferr = %8e, fhelm = %.5e\n", step, o.nsteps, ferr, fhelm);
@@ -741,7 +769,12 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
                 if (o.nsout > 0) std::printf("Updating cell dimensions: serr = %.5e\n", serr);
             }
         }
-        s.errors(&ferr, &fhelm);
+        s.errors_launch();
+        if (queue && step + 1 < o.nsteps) {
+            s.residuals();
+            s.res_queued = true;
+        }
+        s.errors_read(&ferr, &fhelm);
         if (std::isnan(ferr)) throw Error(-5, "Diverging field updater -- breaking.");
         conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
         if (conv) break;
