@@ -229,6 +229,8 @@ struct Momentum : Updater {
     ~Momentum();
     void setup(System* s) override;
     void step() override;
+    // NAG evaluates the residuals at a look-ahead point, the others at the fields as they are
+    bool starts_with_residuals() const override { return method != 3; }
 };
 
 struct Domain {

@@ -748,7 +748,7 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
     bool conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
     // The next step's residuals! are queued before the host reads this step's errors
     // when nothing can change the fields or the cell in between: an updater whose step
-    // starts with residuals!, a fixed cell, and the canonical ensemble (the grand free
+    // starts with residuals! (Anderson, and Momentum other than NAG), a fixed cell, and the canonical ensemble (the grand free
     // energy reads device scalars that the queued pass would overwrite).  A converged
     // system then takes the queued pass as its final residuals!.
     const bool queue = u.starts_with_residuals() && !vcell && s.canonical;
