@@ -142,6 +142,18 @@ def batched_scft(P, count, plain_sample):
     r2 = b.scft(mask=mask, **PH2)
     t_batch = time.perf_counter() - t0
     solved = [i for i, r in enumerate(r2) if r.converged]
+    fhelm32 = [r.fhelm for r in r2]
+    del b, systems
+    # FP64 reference for every instance: the same protocol batched in FP64 (the batch
+    # follows each system's one-at-a-time trajectory to rounding, tested)
+    sys64 = [sweep_system(P, c, "fp64") for c in cfg]
+    b64 = P.Batch(sys64)
+    q1 = b64.scft(**PH1)
+    q2 = b64.scft(mask=[int(r.converged) for r in q1], **PH2)
+    both = [i for i in solved if q2[i].converged]
+    all_errs = [abs(fhelm32[i] - q2[i].fhelm) for i in both]
+    n64 = sum(1 for r in q2 if r.converged)
+    del b64, sys64
     # plain FP64 path, one instance at a time, same protocol
     t_plain, n_plain_solved, errs = 0.0, 0, []
     for i, c in enumerate(cfg[:plain_sample]):
@@ -170,8 +182,13 @@ def batched_scft(P, count, plain_sample):
         "plain_fp64_sample": plain_sample, "plain_fp64_solved": n_plain_solved,
         "plain_fp64_seconds": t_plain, "plain_solves_per_s": plain_rate,
         "speedup_vs_plain": batch_rate / plain_rate if plain_rate > 0 else None,
-        "fhelm_abs_err_fp32_vs_fp64": {"max": max(errs) if errs else None,
-                                       "n": len(errs)},
+        "fhelm_abs_err_fp32_vs_fp64": {
+            "max": max(all_errs) if all_errs else None,
+            "median": float(np.median(all_errs)) if all_errs else None, "n": len(all_errs),
+            "reference": "the same protocol batched in FP64, every instance both runs solve"},
+        "fp64_batched_solved": n64,
+        "fhelm_abs_err_vs_plain_sample": {"max": max(errs) if errs else None,
+                                          "n": len(errs)},
         "protocol": "scft!(Anderson iter_euler=1e9 lam_euler=0.1, ftol 1e-2) then "
                     "scft!(Anderson nhist=10, ftol 1e-6); the batched path runs the second "
                     "phase in FP32 to ftol 1e-5 and polishes in FP64 to 1e-6; solved = "
@@ -181,10 +198,16 @@ def batched_scft(P, count, plain_sample):
 
 # HBM bytes per lane and launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
 # 32-lane dispatch, FETCH x 2.0 and WRITE x 1.0 from a plain copy kernel of known size),
-# per xy-slab form: 3 the shuffle-exchange kernel (default), 2 the 512-thread LDS kernel
+# per xy-slab form: 3 the shuffle-exchange kernel (default), 2 the 512-thread LDS kernel.
+# Stored from the counter runs recorded in profiles/, not measured by this script.
 TRAFFIC_PER_LANE = {("xy-slab pass", 3): (2 * 31170.5 + 66327.6) * 1024 / 32,
                     ("xy-slab pass", 2): (2 * 31193.8 + 66228.7) * 1024 / 32,
                     ("z-pencil pass", None): (2 * 17091.8 + 33792.0) * 1024 / 32}
+
+
+TRAFFIC_SOURCE = ("stored: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this kernel "
+                  "form at this workload (profiles/propagator_64cubed_32inst_fp32.md), "
+                  "scaled by a copy kernel's reading; L2-to-memory bytes, not DRAM")
 
 
 def traffic_per_launch(kernel, args, slab_lanes, pen_lanes, slab_form):
@@ -195,6 +218,48 @@ def traffic_per_launch(kernel, args, slab_lanes, pen_lanes, slab_form):
     form = 3 if slab_form < 0 else slab_form
     per = TRAFFIC_PER_LANE.get((kernel, form))
     return per * slab_lanes if per is not None else None
+
+
+def precision_error(P, systems32):
+    """FP32 propagator against FP64 on the same fields: the FP32 systems' current
+    densities, Q and fhelm against a fresh FP64 evaluation (max over the systems)."""
+    out = {"rho_max_abs": 0.0, "Q_rel": 0.0, "fhelm_abs": 0.0, "instances": len(systems32)}
+    for s32 in systems32:
+        s64, ch64 = build(P, "fp64")
+        for m in s64.mids:
+            s64.fields[m] = np.array(s32.fields[m])
+        s64.density_update()
+        q32, q64 = s32.species_scalars(0)[0], s64.species_scalars(0)[0]
+        out["Q_rel"] = max(out["Q_rel"], abs(q32 - q64) / abs(q64))
+        out["fhelm_abs"] = max(out["fhelm_abs"], abs(s32.free_energy() - s64.free_energy()))
+        for m in s64.mids:
+            d = float(np.abs(np.asarray(s32.density[m]) - np.asarray(s64.density[m])).max())
+            out["rho_max_abs"] = max(out["rho_max_abs"], d)
+        del s64
+    return out
+
+
+def fp64_rate(P, args, nsteps_mde):
+    """The headline workload with the FP64 propagator (the plain mode): MDE steps/s of
+    the same batch of instances, replayed from its captured graph."""
+    systems = []
+    for k in range(args.instances):
+        s, ch = build(P, "fp64", args.seed + k)
+        for m, w in seeded_fields(s.shape, args.seed + k).items():
+            s.fields[m] = w
+        systems.append(s)
+    batch = P.Batch(systems)
+    for _ in range(args.warmup):
+        batch.density_update()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.density_update()
+    dt = time.perf_counter() - t0
+    value = nsteps_mde * args.instances * args.steps / dt
+    n, nk = GRID ** 3, (GRID // 2 + 1) * GRID * GRID
+    return {"value": value, "unit": "MDE steps/s", "ms_per_step": dt * 1e3 / args.steps,
+            "step_roofline_frac": value * 8 * (3 * n + 4 * nk) / HBM_PEAK,
+            "instances": args.instances}
 
 
 def single_instance(P, args, nsteps_mde):
@@ -299,6 +364,60 @@ def grad_workload(P, args):
             "density_ms": dt_d * 1e3, "gradient_over_density": dt_g / dt_d}
 
 
+# config-2 SCFT: MomentumUpdater(:MOM, lam = 0.1) + DomainUpdater(nskip = 100, nper = 10),
+# ftol = stol = 1e-5 (the reference's Anderson mixing diverges from this seed)
+SCFT64 = dict(lam=0.1, nskip=100, nper=10, ftol=1e-5, stol=1e-5, nsteps=5000)
+
+
+def scft64_run(P, precision, seed, ftol):
+    s, ch = build(P, precision)
+    for m, w in seeded_fields(s.shape, seed).items():
+        s.fields[m] = w
+    s.residuals_update()
+    s.synchronize()
+    t0 = time.perf_counter()
+    r = P.scft(s, P.MomentumUpdater(method="MOM", lam=SCFT64["lam"]),
+               domain=P.DomainUpdater(nskip=SCFT64["nskip"], nper=SCFT64["nper"]),
+               opts=P.SCFTOptions(nsteps=SCFT64["nsteps"], ftol=ftol, stol=SCFT64["stol"]))
+    s.synchronize()
+    return s, r, time.perf_counter() - t0
+
+
+def scft64_workload(P, args):
+    """The 64^3 variable-cell SCFT from the BCC-biased seed, to convergence, in the FP64
+    plain mode and with the FP32 propagator (stopped at its residual floor, ftol 2e-4),
+    with the FP32 run's and the FP32 propagator's errors against FP64."""
+    s64, r64, t64 = scft64_run(P, "fp64", args.seed, SCFT64["ftol"])
+    a64, f64 = float(s64.cell.params[0]), s64.free_energy()
+    rho64 = {m: np.array(s64.density[m]) for m in s64.mids}
+    s64.set_precision("fp32")
+    s64.density_update()
+    at = {"fhelm_abs": abs(s64.free_energy() - f64),
+          "rho_max_abs": max(float(np.abs(np.asarray(s64.density[m]) - rho64[m]).max())
+                             for m in s64.mids)}
+    del s64
+    s32, r32, t32 = scft64_run(P, "fp32", args.seed, 2e-4)
+    a32 = float(s32.cell.params[0])
+    amp = lambda r: np.abs(np.fft.rfftn(np.asarray(r)))
+    err = {"fhelm_abs": abs(r32.fhelm - r64.fhelm), "a_rel": abs(a32 - a64) / a64,
+           "rho_max_abs": max(float(np.abs(np.asarray(s32.density[m]) - rho64[m]).max())
+                              for m in s32.mids),
+           "rho_amp_rel": max(float(np.abs(amp(s32.density[m]) - amp(rho64[m])).max() /
+                                    amp(rho64[m]).max()) for m in s32.mids)}
+    res = lambda r, t, a: {"seconds": t, "nsteps": r.nsteps, "converged": r.converged,
+                           "ferr": r.ferr, "serr": r.serr, "fhelm": r.fhelm, "a": a}
+    return {"metric": "64^3 variable-cell SCFT solves/s (FP64 plain mode)", "value": 1.0 / t64,
+            "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": t64 * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded 0.1*N(0,1) + BCC cosine bias)",
+            "config": {"workload": "64^3 cubic AB diblock N=100 fA=0.35 chiN=20 Ns=102, "
+                                   "a0=4 Rg; scft! with MomentumUpdater(MOM, lam=0.1) + "
+                                   "DomainUpdater(nskip=100, nper=10), ftol=stol=1e-5"},
+            "fp64": res(r64, t64, a64), "fp32": res(r32, t32, a32),
+            "fp32_run_error_vs_fp64": err, "fp32_propagator_at_fp64_state": at}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,14 +430,17 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--sweep-instances", type=int, default=256)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--schedule", default=None, metavar="SLAB,PENCIL",
                     help="density64: fix the 64^3 slab and pencil kernel forms (the numbers "
                          "of polyfields_hip.schedule_set) instead of autotuning")
-    ap.add_argument("--workload", default="density64", choices=["density64", "blend128", "grad64"],
-                    help="density64: the headline (64^3 batch); blend128 / grad64: the large "
-                         "blend and the free-energy gradient, one JSON line each")
+    ap.add_argument("--workload", default="density64",
+                    choices=["density64", "blend128", "grad64", "scft64"],
+                    help="density64: the headline (64^3 batch); blend128 / grad64 / scft64: "
+                         "the large blend, the free-energy gradient and the 64^3 "
+                         "variable-cell SCFT, one JSON line each")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT backend runs on a single MI355X "
@@ -328,7 +450,8 @@ def main():
     import polyfields_hip as P
 
     if args.workload != "density64":
-        fn = blend_workload if args.workload == "blend128" else grad_workload
+        fn = {"blend128": blend_workload, "grad64": grad_workload,
+              "scft64": scft64_workload}[args.workload]
         print(json.dumps(fn(P, args)))
         return
 
@@ -411,11 +534,16 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic_per_launch(dom[0], args, slab_lanes, pen_lanes,
                                                    slab_form),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "bytes_per_launch": dom[1], "avg_launch_us": dom[2] * 1e6},
         "step_roofline_frac": value * b_step / HBM_PEAK,
         "kernel_avg_us": {"slab": slab_avg * 1e6, "pencil": pen_avg * 1e6},
     }
+    if args.precision == "fp32":
+        line["fp32_error_vs_fp64"] = precision_error(P, systems[:4])
     del batch, systems
+    if args.precision == "fp32" and not args.no_fp64:
+        line["fp64"] = fp64_rate(P, args, nsteps_mde)
     # the same workload with one instance in flight (two lanes per launch)
     if not args.no_single:
         line["single_instance_steps_per_s"] = single_instance(P, args, nsteps_mde)

@@ -26,6 +26,7 @@
  *   pf_system_get_propagator  species.q[s] / species.qc[s]                      species/multiblock.jl:80-81
  *   pf_system_species_scalars species.Q, sys.phi_species, sys.mu_species         system/system.jl:44-45
  *   pf_system_cell .......... cell.params / update!(cell)                       system/cell.jl:96-109
+ *   pf_system_add_cell ...... add_cell!(sys, cell)                              system/system.jl:102-106
  *   pf_validate ............. validate(sys)                                     system/system.jl:192-237
  *   pf_density .............. density!(sys)                                     system/system.jl:266-289
  *   pf_residuals ............ residuals!(sys)                                   system/system.jl:317-340
@@ -41,6 +42,7 @@
  *   pf_domain_setup/step .... setup!(d, sys) / step!(d; nsteps, tol)            updaters/domain.jl:32-92
  *   pf_domain_stress ........ domain.stress / stress_error(d)                   updaters/domain.jl:30,44-48
  *   pf_scft ................. scft!(sys, updater; domain, opts) -> SCFTResults  simulate.jl:60-148
+ *   pf_scft_save ............ the same, with savefields every nsave steps      simulate.jl:124-128
  *   pf_cell_geometry ........ make_basis / basis! / dbasis! / ddbasis!          system/basis.jl:11-116,
  *                                                                               system/cell.jl:113-258
  *   pf_chain_grid ........... chain_grid(N_block, Ns)                           species/base.jl:27-56
@@ -80,7 +82,7 @@ typedef struct pf_domain pf_domain;
 typedef struct {
     int nsteps;    /* SCFTOptions.nsteps (default 100)  */
     int nsout;     /* print progress every nsout steps (0: quiet) */
-    int nsave;     /* accepted, unused: field files are the caller's business */
+    int nsave;     /* pf_scft_save calls its save function every nsave steps (0: never) */
     double ftol;   /* default 1e-5 */
     double stol;   /* default 1e-4 */
 } pf_scft_options;
@@ -140,6 +142,11 @@ int pf_system_get_propagator(pf_system* sys, int species, int s, int backward, d
 int pf_system_species_scalars(pf_system* sys, int species, double* Q, double* phi, double* mu);
 /* set != 0: write params (length nparams) and update!(cell); else read them. */
 int pf_system_cell(pf_system* sys, double* params, int nparams, int set);
+/* add_cell!(sys, UnitCell(dim, crystal, params...)): replace the system's cell, crystal
+   system included (the dimension must match the grid's); as fieldinit!(sys, fpath;
+   load_cell = true) does with the cell of a field file. */
+int pf_system_add_cell(pf_system* sys, int dim, const char* crystal, const double* params,
+                       int nparams);
 int pf_system_synchronize(pf_system* sys);
 /* Switch the propagator precision (PF_PRECISION_FP64 / _FP32).  Fields, densities and
    every sum are float64 in both modes, so the state carries over; the q stacks are
@@ -187,7 +194,10 @@ int pf_system_set_checkpoint(pf_system* sys, int K);
    block keeps its number of contour steps and ds_b = v_b N_b / steps_b. */
 int pf_species_set_blocks(pf_system* sys, int isp, const double* N_block, int nblk);
 
-/* Field updaters.  method: Euler 0 PEC, 1 PECE; Momentum 0 MOM, 1 RMS, 2 ADAM, 3 NAG. */
+/* Field updaters.  method: Euler 0 PEC, 1 PECE; Momentum 0 MOM, 1 RMS, 2 ADAM, 3 NAG.
+   Anderson accepts any nhist >= 1, as the reference does (its default is 50): depths up
+   to 16 reduce the Gram matrix in one pass over the history, deeper ones in 8 x 8 tiles;
+   the history holds nhist + 1 field and residual grids in device memory. */
 int pf_updater_anderson(pf_updater** out, int nhist, int iter_euler, double lam_euler);
 int pf_updater_euler(pf_updater** out, int method, double lam);
 int pf_updater_momentum(pf_updater** out, int method, double lam, double gamma, double beta1,
@@ -210,6 +220,15 @@ int pf_domain_destroy(pf_domain* d);
 /* Driver.  domain may be NULL (fixed cell). */
 int pf_scft(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
             pf_scft_results* res);
+/* pf_scft with field saving (src/simulate.jl:124-128): save(user, step) is called where
+   the reference calls savefields(sys, opts.savepath) -- after step `step` of the loop
+   when nsave > 0 and step % nsave == 0 and the run has not converged -- and once more
+   with step = -1 after the final residuals! (the reference's own final write names an
+   option that SCFTOptions lacks, src/simulate.jl:140-142).  The callback may read the
+   system's grids; it must not change them.  NULL save: pf_scft. */
+typedef void (*pf_save_fn)(void* user, int step);
+int pf_scft_save(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
+                 pf_scft_results* res, pf_save_fn save, void* user);
 
 /* Batches.  Systems of one precision on one device advance together: every propagator
    launch covers the lanes of all systems (grouped by grid size), every system-level

@@ -298,6 +298,30 @@ int pf_system_cell(pf_system* sys, double* params, int nparams, int set) {
     });
 }
 
+int pf_system_add_cell(pf_system* sys, int dim, const char* crystal, const double* params,
+                       int nparams) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(crystal && params && nparams > 0, "cell arguments");
+        if (dim != s.dim)
+            throw pf::Error(-2, "Number of non-singleton dimensions in System and Cell do not match!");
+        const int np = pf::crystal_nparams(dim, crystal);
+        if (np < 0)
+            throw pf::Error(-2, std::string("Unrecognized crystal system `") + crystal +
+                                    "` for dim = " + std::to_string(dim) + ".");
+        need(nparams == np, "cell parameter count for the crystal system");
+        pf::Cell c;
+        c.dim = dim;
+        c.crystal = crystal;
+        c.params.assign(params, params + nparams);
+        c.version = s.cell.version;   // update() moves it past every table built so far
+        c.update();
+        s.sync();
+        s.cell = c;
+        s.dirty_species = 1;
+    });
+}
+
 int pf_system_synchronize(pf_system* sys) {
     return guard([&] { S(sys).sync(); });
 }
@@ -504,10 +528,17 @@ int pf_domain_destroy(pf_domain* d) {
 
 int pf_scft(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
             pf_scft_results* res) {
+    return pf_scft_save(sys, u, d, opts, res, nullptr, nullptr);
+}
+
+int pf_scft_save(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_options* opts,
+                 pf_scft_results* res, pf_save_fn save, void* user) {
     return guard([&] {
         need(u && u->u && opts && res, "scft arguments");
         pf::ScftOptions o{opts->nsteps, opts->nsout, opts->nsave, opts->ftol, opts->stol};
-        pf::ScftResults r = pf::scft(S(sys), *u->u, d ? d->d : nullptr, o);
+        std::function<void(int)> cb;
+        if (save) cb = [save, user](int step) { save(user, step); };
+        pf::ScftResults r = pf::scft(S(sys), *u->u, d ? d->d : nullptr, o, cb);
         res->time = r.time;
         res->nsteps = r.nsteps;
         res->converged = r.converged;
@@ -624,7 +655,7 @@ int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int 
                   double ftol, const int* mask, pf_scft_results* results) {
     return guard([&] {
         need(results, "results");
-        need(nhist >= 1 && nhist <= PF_MAXHIST, "nhist in 1..16");
+        need(nhist >= 1, "nhist must be at least 1");
         auto r = B(b).scft_anderson(nhist, iter_euler, lam_euler, nsteps, ftol, mask);
         for (size_t i = 0; i < r.size(); ++i) {
             results[i].time = r[i].time;

@@ -81,8 +81,7 @@ Batch::Batch(const std::vector<System*>& s) : sys(s) {
     sums = (double*)dev_alloc(sizeof(double) * 256 * ns);
     csums = (double*)dev_alloc(sizeof(double) * kCompact * ns);
     PF_HIP(hipHostMalloc((void**)&hcsums, sizeof(double) * kCompact * ns, hipHostMallocDefault));
-    d_step = (StepTask*)dev_alloc(sizeof(StepTask) * ns);
-    PF_HIP(hipHostMalloc((void**)&h_step, sizeof(StepTask) * ns, hipHostMallocDefault));
+    alloc_step_table(PF_MAXHIST);
     eng = make_batch_engine(*this);
     if (ns >= 64) {
         const unsigned hw = std::thread::hardware_concurrency();
@@ -98,12 +97,52 @@ Batch::~Batch() {
     eng.reset();
     dev_free(sums);
     dev_free(csums);
+    dev_free(gpart);
     dev_free(d_step);
     dev_free(d_act);
     if (hcsums) (void)hipHostFree(hcsums);
     if (h_step) (void)hipHostFree(h_step);
     if (sums_ev) (void)hipEventDestroy(sums_ev);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Step table for histories of depth nhist (kept if the current one is large enough),
+// and for depths above PF_MAXHIST the tiled-Gram scratch and a compact area of
+// gram_tiles(nhist) * 64 values per system.  Called with the stream idle.
+void Batch::alloc_step_table(int nhist) {
+    const size_t ns = sys.size();
+    if (!d_step || nhist > coef_stride) {
+        dev_free(d_step);
+        if (h_step) PF_HIP(hipHostFree(h_step));
+        d_step = nullptr;
+        h_step = nullptr;
+        coef_stride = std::max(nhist, PF_MAXHIST);
+        const size_t bytes = (sizeof(StepTask) + sizeof(double) * coef_stride) * ns;
+        d_step = (StepTask*)dev_alloc(bytes);
+        PF_HIP(hipHostMalloc((void**)&h_step, bytes, hipHostMallocDefault));
+    }
+    if (nhist > PF_MAXHIST) {
+        const int nv = gram_tiles(nhist) * 64;
+        if (nv > ncompact) {
+            dev_free(csums);
+            if (hcsums) PF_HIP(hipHostFree(hcsums));
+            csums = nullptr;
+            hcsums = nullptr;
+            ncompact = nv;
+            csums = (double*)dev_alloc(sizeof(double) * (size_t)ncompact * ns);
+            PF_HIP(hipHostMalloc((void**)&hcsums, sizeof(double) * (size_t)ncompact * ns,
+                                 hipHostMallocDefault));
+        }
+        size_t Nmax = 0;
+        for (System* x : sys) Nmax = std::max(Nmax, x->n * x->nmon());
+        const int parts = gram_tiled_parts(Nmax);
+        if (parts != gparts || !gpart) {
+            dev_free(gpart);
+            gpart = nullptr;
+            gparts = parts;
+            gpart = (double*)dev_alloc(sizeof(double) * (size_t)nv * gparts * ns);
+        }
+    }
 }
 
 void Batch::drop_graphs() {
@@ -263,6 +302,9 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
         s->sync();
     }
     const std::vector<int> run = act;
+    PF_HIP(hipStreamSynchronize(stream));
+    alloc_step_table(nhist);
+    const bool deep = nhist > PF_MAXHIST;
     residuals(act);
     and_state.clear();
     for (size_t i = 0; i < ns; ++i) {
@@ -323,17 +365,22 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
             s.lam = lam;
             s.lam_euler = A.lam_euler;
             s.r_out = A.rslot(ih);
-            for (int nn = 0; nn < s.h; ++nn) {
-                s.fh.p[nn] = A.wslot(ih - (nn + 1));
-                s.rh.p[nn] = A.rslot(ih - (nn + 1));
-            }
+            if (s.h > 0) s.ring = A.ring(ih);
+            s.C = d_coef() + k * (size_t)coef_stride;
             s.f_iter = A.wslot(ih);
             s.f_out = (iter < nhist + 1) ? A.wslot(iter) : A.wslot(0);
         }
+        // the StepTasks; the coefficients follow after the host solves below
         PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
         if (!euler) {
-            const int nu = ih * (ih + 1) / 2, nv = nu + ih;
-            launch_gram_b(ih, d_act, d_step, (int)act.size(), max_parts(tasks, true), csums, stream);
+            const int nu = ih * (ih + 1) / 2;
+            const int nv = deep ? gram_tiles(ih) * 64 : nu + ih;
+            if (deep)
+                launch_gram_tiled_b(ih, d_act, d_step, (int)act.size(), gparts, gpart, csums,
+                                    stream);
+            else
+                launch_gram_b(ih, d_act, d_step, (int)act.size(), max_parts(tasks, true), csums,
+                              stream);
             PF_HIP(hipMemcpyAsync(hcsums, csums, sizeof(double) * nv * act.size(), hipMemcpyDeviceToHost, stream));
             PF_HIP(hipStreamSynchronize(stream));
             // each system's solve is independent: large batches split them over host threads
@@ -341,26 +388,37 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
                 std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
                 for (size_t k = k0; k < k1; ++k) {
                     const double* g = hcsums + k * (size_t)nv;
-                    int q = 0;
-                    for (int m = 0; m < ih; ++m)
-                        for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
-                    for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
+                    double* hc = h_coef() + k * (size_t)coef_stride;
+                    if (deep) {
+                        for (int m = 0; m < ih; ++m) {
+                            for (int nn = m; nn < ih; ++nn)
+                                U[m * ih + nn] = U[nn * ih + m] = g[gram_tile_index(m, nn)];
+                            V[m] = g[gram_tile_index(m, ih)];
+                        }
+                    } else {
+                        int q = 0;
+                        for (int m = 0; m < ih; ++m)
+                            for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
+                        for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
+                    }
                     if (!lu_inverse(ih, U.data(), inv.data())) {
                         // scft! raises on a singular Gram matrix (anderson.jl:203-208); in a
                         // batch the system is reported as failed and dropped
                         res[act[k]].diverged = 1;
-                        for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = 0.0;
+                        for (int nn = 0; nn < ih; ++nn) hc[nn] = 0.0;
                         continue;
                     }
                     matvec(ih, inv.data(), V.data(), C.data());
-                    for (int nn = 0; nn < ih; ++nn) h_step[k].C[nn] = C[nn];
+                    for (int nn = 0; nn < ih; ++nn) hc[nn] = C[nn];
                 }
             };
             if (pool && act.size() >= 64)
                 pool->run(act.size(), solve);
             else
                 solve(0, act.size());
-            PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
+            PF_HIP(hipMemcpyAsync(const_cast<double*>(d_coef()), h_coef(),
+                                  sizeof(double) * coef_stride * act.size(),
+                                  hipMemcpyHostToDevice, stream));
         }
         size_t Nmax = 0;
         for (int i : act) Nmax = std::max(Nmax, sys[i]->n * sys[i]->nmon());

@@ -42,11 +42,11 @@ struct SysTask {
 
 // Per-iteration Anderson data of one system.
 struct StepTask {
-    HistPtrs fh, rh;     // logical history slots ih-1 .. ih-h (fields, residuals)
+    HistRing ring;       // the history ring (logical slots ih-1 .. ih-h)
     const double* f_iter;
     double* f_out;       // history slot receiving the new field
     double* r_out;       // history slot receiving the residual
-    double C[PF_MAXHIST];
+    const double* C;     // h mixing coefficients (device copy of the step table)
     double lam, lam_euler;
     int h, euler;
 };
@@ -59,6 +59,10 @@ void launch_residuals_b(const SysTask* d, int nsys, int maxparts, size_t nmax, h
 void launch_sys_reduce_b(const SysTask* d, int nsys, int maxparts, double* compact, hipStream_t st);
 void launch_gram_b(int h, const SysTask* d, const StepTask* s, int nsys, int maxparts,
                    double* compact, hipStream_t st);
+// any depth h: tiled Gram, partial[(sys * parts + part) * tiles * 64] with `parts` the
+// largest gram_tiled_parts of the systems, compact[sys][tiles * 64]
+void launch_gram_tiled_b(int h, const SysTask* d, const StepTask* s, int nsys, int parts,
+                         double* partial, double* compact, hipStream_t st);
 void launch_mix_b(const SysTask* d, const StepTask* s, int nsys, size_t Nmax, hipStream_t st);
 
 class BatchEngineBase {
@@ -121,13 +125,22 @@ class Batch {
     std::unique_ptr<BatchEngineBase> eng;
     double* sums = nullptr;          // nsys * 256 device doubles
     // per active system, in active-set order, the values the host reads (the five system
-    // sums, or the Gram entries): device copy and pinned mirror, nsys * kCompact doubles
+    // sums, or the Gram entries): device copy and pinned mirror, nsys * ncompact doubles
     static constexpr int kCompact = PF_MAXHIST * (PF_MAXHIST + 3) / 2;
+    int ncompact = kCompact;
     double* csums = nullptr;
     double* hcsums = nullptr;
+    double* gpart = nullptr;         // tiled-Gram partial sums (histories above PF_MAXHIST)
+    int gparts = 0;
     SysTask* d_sys = nullptr;        // all systems' descriptors
+    // step table: nsys StepTasks, then nsys * coef_stride mixing coefficients; pinned
+    // staging and device copy, uploaded as one block
     StepTask* d_step = nullptr;
-    StepTask* h_step = nullptr;      // pinned staging
+    StepTask* h_step = nullptr;
+    int coef_stride = PF_MAXHIST;
+    double* h_coef() const { return reinterpret_cast<double*>(h_step + sys.size()); }
+    const double* d_coef() const { return reinterpret_cast<const double*>(d_step + sys.size()); }
+    void alloc_step_table(int nhist);
     std::vector<SysTask> tasks;
     std::vector<int> act_cache;      // active set the compacted descriptors were built for
     SysTask* d_act = nullptr;        // descriptors of the active systems, compacted

@@ -209,11 +209,15 @@ __global__ void __launch_bounds__(RB) k_gram_b(const SysTask* __restrict__ tasks
     for (int k = 0; k < NU + H; ++k) acc[k] = 0.0;
     const size_t N = t->n * t->ra.nmon;
     const double* r = t->ra.res;
+    const HistRing g = st->ring;
+    const double* hp[H];
+#pragma unroll
+    for (int m = 0; m < H; ++m) hp[m] = g.r + ring_off(g, m);
     for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < N; i += (size_t)t->gparts * RB) {
         const double ri = r[i];
         double d[H];
 #pragma unroll
-        for (int m = 0; m < H; ++m) d[m] = ri - st->rh.p[m][i];
+        for (int m = 0; m < H; ++m) d[m] = ri - hp[m][i];
         int k = 0;
 #pragma unroll
         for (int m = 0; m < H; ++m)
@@ -239,8 +243,71 @@ void launch_gram_b(int h, const SysTask* d, const StepTask* s, int nsys, int max
         PF_G(1) PF_G(2) PF_G(3) PF_G(4) PF_G(5) PF_G(6) PF_G(7) PF_G(8)
         PF_G(9) PF_G(10) PF_G(11) PF_G(12) PF_G(13) PF_G(14) PF_G(15) PF_G(16)
 #undef PF_G
-        default: throw Error(-2, "Anderson history depth above 16 is not supported");
+        default: throw Error(-2, "launch_gram_b: history depth above PF_MAXHIST");
     }
+    PF_HIP(hipGetLastError());
+}
+
+// Any depth: 8 x 8 tiles of the Gram matrix of x_m = r - r_m (m < h) and x_h = r, as
+// k_gram_tiled does for one system; grid (parts, tiles, systems).
+__global__ void __launch_bounds__(RB) k_gram_tiled_b(const SysTask* __restrict__ tasks,
+                                                     const StepTask* __restrict__ steps,
+                                                     double* partial) {
+    const SysTask* t = tasks + blockIdx.z;
+    const StepTask* st = steps + blockIdx.z;
+    const HistRing g = st->ring;
+    const int parts = gram_tiled_parts(g.N);
+    if ((int)blockIdx.x >= parts) return;
+    const int h = st->h;
+    const int p = blockIdx.y;
+    int J = 0;
+    while ((J + 1) * (J + 2) / 2 <= p) ++J;
+    const int I = p - J * (J + 1) / 2;
+    const double* r = t->ra.res;
+    const size_t N = g.N;
+    double acc[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) acc[k] = 0.0;
+    auto x = [&](int k, size_t i, double ri) {
+        if (k < h) return ri - g.r[ring_off(g, k) + i];
+        return k == h ? ri : 0.0;
+    };
+    for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < N; i += (size_t)parts * RB) {
+        const double ri = r[i];
+        double a[8], b[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            a[q] = x(8 * I + q, i, ri);
+            b[q] = x(8 * J + q, i, ri);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[q * 8 + u] += a[q] * b[u];
+    }
+    const size_t row = ((size_t)blockIdx.z * gridDim.x + blockIdx.x) * gridDim.y + p;
+    wg_store<64>(acc, partial + row * 64);
+}
+
+// compact[sys * nv + k] = sum over the system's parts (in order) of its partial rows
+__global__ void k_gram_tiled_final_b(const StepTask* __restrict__ steps,
+                                     const double* __restrict__ partial, int maxparts, int nv,
+                                     double* compact) {
+    const double* base = partial + (size_t)blockIdx.x * maxparts * nv;
+    const int parts = gram_tiled_parts(steps[blockIdx.x].ring.N);
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < parts; ++q) s += base[(size_t)q * nv + k];
+        compact[(size_t)blockIdx.x * nv + k] = s;
+    }
+}
+
+void launch_gram_tiled_b(int h, const SysTask* d, const StepTask* s, int nsys, int parts,
+                         double* partial, double* compact, hipStream_t st) {
+    const int nt = gram_tiles(h);
+    hipLaunchKernelGGL(k_gram_tiled_b, dim3(parts, nt, nsys), dim3(RB), 0, st, d, s, partial);
+    hipLaunchKernelGGL(k_gram_tiled_final_b, dim3(nsys), dim3(256), 0, st, s, partial, parts,
+                       nt * 64, compact);
     PF_HIP(hipGetLastError());
 }
 
@@ -263,9 +330,11 @@ __global__ void k_mix_b(const SysTask* __restrict__ tasks, const StepTask* __res
             double WW = w[i];
             double DD = ri;
             const double fi = s->f_iter[i];
+            const HistRing& g = s->ring;
             for (int nn = 0; nn < s->h; ++nn) {
-                WW += s->C[nn] * (s->fh.p[nn][i] - fi);
-                DD += s->C[nn] * (s->rh.p[nn][i] - ri);
+                const size_t off = ring_off(g, nn) + i;
+                WW += s->C[nn] * (g.w[off] - fi);
+                DD += s->C[nn] * (g.r[off] - ri);
             }
             o = WW + s->lam * DD;
         }

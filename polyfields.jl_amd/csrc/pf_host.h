@@ -1,6 +1,7 @@
 // Host-side state of a device-resident FieldSystem and its updaters.
 #pragma once
 
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -201,6 +202,15 @@ struct Anderson : Updater {
     double* hist_w = nullptr;
     int base = 0;                    // physical slot of logical slot 0
     std::vector<double> lastU, lastV, lastC;
+    // histories deeper than PF_MAXHIST: tiled Gram scratch and its pinned host copy,
+    // mixing coefficients (pinned staging and device copy)
+    double* gpart = nullptr;
+    double* gout = nullptr;
+    double* hgram = nullptr;
+    double* hcoef = nullptr;
+    double* dcoef = nullptr;
+    int gparts = 0;
+    HistRing ring(int ih) const;     // the history ring as seen by step ih
     Anderson(int nhist, int iter_euler, double lam_euler);
     ~Anderson();
     double* rslot(int logical) const;
@@ -257,7 +267,12 @@ struct ScftResults {
     int nsteps, converged;
     double ferr, serr, fhelm;
 };
-ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o);
+// save(step): called where the reference writes savefields(sys, opts.savepath) -- every
+// nsave steps inside the loop (src/simulate.jl:124-128) -- and, with step = -1, once for
+// the final state (the reference's own final write names a field SCFTOptions does not
+// have, src/simulate.jl:140-142).  Empty: no saving.
+ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o,
+                 const std::function<void(int)>& save = {});
 
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);

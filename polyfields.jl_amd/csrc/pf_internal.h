@@ -11,7 +11,8 @@
 #define PF_MAXPASS 16    // radix passes per transform
 #define PF_MAXMON 8      // monomer types per system
 #define PF_MAXPAR 6      // cell parameters
-#define PF_MAXHIST 16    // Anderson history depth
+#define PF_MAXHIST 16    // Anderson history depth served by the one-pass Gram kernels;
+                         // deeper histories take the tiled Gram (any depth)
 
 namespace pf {
 
@@ -231,6 +232,36 @@ struct MixCoef {
     double C[PF_MAXHIST];
 };
 
+// Anderson history as a ring of `nslots` slots of N doubles each: logical slot m of a
+// Gram or mix (m = 0 the newest earlier entry, m = h - 1 the oldest) is physical slot
+// (start - m) % nslots, with start >= h - 1.
+struct HistRing {
+    const double* r;     // residual slots
+    const double* w;     // field slots
+    size_t N;
+    int nslots, start;
+};
+__host__ __device__ inline size_t ring_off(const HistRing& g, int m) {
+    return (size_t)((g.start - m) % g.nslots) * g.N;
+}
+
+// Tiled Gram of the vectors x_m = r - r_m (m < h) and x_h = r: 8 x 8 tiles (I <= J) of
+// the (h+1) x (h+1) matrix, tile p = J (J + 1) / 2 + I, entry (a, b) at p * 64 + a * 8 + b.
+inline int gram_tiles(int h) {
+    const int nb = (h + 1 + 7) / 8;
+    return nb * (nb + 1) / 2;
+}
+// workgroups per tile row of a system of N history points (one per 8192 points, at most
+// 64): a function of N only, so that a batch sums in the same order as one system
+__host__ __device__ inline int gram_tiled_parts(size_t N) {
+    const size_t p = N / 8192;
+    return (int)(p < 1 ? 1 : (p > 64 ? 64 : p));
+}
+inline int gram_tile_index(int i, int j) {   // entry (i, j), i <= j, of the Gram matrix
+    const int I = i / 8, J = j / 8;
+    return (J * (J + 1) / 2 + I) * 64 + (i % 8) * 8 + (j % 8);
+}
+
 int reduce_parts(size_t n);
 template <typename T>
 void launch_lw_table(T* out, const double* w, double coef, size_t n, hipStream_t st);
@@ -258,6 +289,13 @@ void launch_copy(double* dst, const double* src, size_t n, hipStream_t st);
 void launch_axpy(double* w, const double* r, double lam, size_t n, hipStream_t st);
 void launch_gram(int h, const double* r, const HistPtrs& hp, size_t N, double* partial,
                  int nparts, double* out, hipStream_t st);
+// any depth: the tiled Gram (partial: gram_tiled_parts(N) * gram_tiles(h) * 64 doubles;
+// out: gram_tiles(h) * 64 values) and the mix with coefficients C[h] in device memory
+void launch_gram_tiled(int h, const double* r, const HistRing& g, double* partial, double* out,
+                       hipStream_t st);
+void launch_anderson_mix_ring(int h, double* w, const double* r, const HistRing& g,
+                              const double* f_iter, const double* C, double lam, double* f_out,
+                              hipStream_t st);
 void launch_anderson_mix(int h, double* w, const double* r, const HistPtrs& fh,
                          const HistPtrs& rh, const double* f_iter, const MixCoef& c, double lam,
                          size_t N, double* f_out, hipStream_t st);

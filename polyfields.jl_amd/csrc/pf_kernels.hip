@@ -427,6 +427,79 @@ void launch_gram(int h, const double* r, const HistPtrs& hp, size_t N, double* p
     PF_HIP(hipGetLastError());
 }
 
+// Anderson Gram for any history depth h (src/updaters/anderson.jl:166-200): the
+// (h+1) x (h+1) Gram matrix of x_m = r - r_m (m < h) and x_h = r, by 8 x 8 tiles.
+// Workgroup (part, tile) sums its tile over a grid-stride share of the points; the
+// entries past h of the last tile row are zero.  U_mn is entry (m, n), V_m entry (m, h).
+__device__ __forceinline__ double gram_vec(const double* __restrict__ r, const HistRing& g,
+                                           int h, int k, size_t i, double ri) {
+    if (k < h) return ri - g.r[ring_off(g, k) + i];
+    return k == h ? ri : 0.0;
+}
+
+__global__ void __launch_bounds__(RB) k_gram_tiled(const double* __restrict__ r, HistRing g,
+                                                   int h, double* partial) {
+    const int p = blockIdx.y;
+    int J = 0;
+    while ((J + 1) * (J + 2) / 2 <= p) ++J;
+    const int I = p - J * (J + 1) / 2;
+    double acc[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) acc[k] = 0.0;
+    const size_t N = g.N;
+    for (size_t i = blockIdx.x * (size_t)RB + threadIdx.x; i < N;
+         i += (size_t)gridDim.x * RB) {
+        const double ri = r[i];
+        double a[8], b[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            a[t] = gram_vec(r, g, h, 8 * I + t, i, ri);
+            b[t] = gram_vec(r, g, h, 8 * J + t, i, ri);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc[t * 8 + u] += a[t] * b[u];
+    }
+    wg_reduce_store<64>(acc, partial + ((size_t)blockIdx.x * gridDim.y + p) * 64);
+}
+
+void launch_gram_tiled(int h, const double* r, const HistRing& g, double* partial, double* out,
+                       hipStream_t st) {
+    const int nt = gram_tiles(h), nparts = gram_tiled_parts(g.N);
+    hipLaunchKernelGGL(k_gram_tiled, dim3(nparts, nt), dim3(RB), 0, st, r, g, h, partial);
+    launch_final_reduce(partial, nparts, nt * 64, out, st);
+    PF_HIP(hipGetLastError());
+}
+
+__global__ void k_anderson_mix_ring(int h, double* w, const double* __restrict__ r, HistRing g,
+                                    const double* __restrict__ f_iter,
+                                    const double* __restrict__ C, double lam, double* f_out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < g.N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double WW = w[i];
+        double DD = r[i];
+        const double fi = f_iter[i], ri = r[i];
+        for (int nn = 0; nn < h; ++nn) {
+            const size_t o = ring_off(g, nn) + i;
+            WW += C[nn] * (g.w[o] - fi);
+            DD += C[nn] * (g.r[o] - ri);
+        }
+        const double o = WW + lam * DD;
+        w[i] = o;
+        if (f_out) f_out[i] = o;
+    }
+}
+
+void launch_anderson_mix_ring(int h, double* w, const double* r, const HistRing& g,
+                              const double* f_iter, const double* C, double lam, double* f_out,
+                              hipStream_t st) {
+    int gr = (int)std::min<size_t>((g.N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_anderson_mix_ring, dim3(gr), dim3(256), 0, st, h, w, r, g, f_iter, C,
+                       lam, f_out);
+    PF_HIP(hipGetLastError());
+}
+
 // Anderson mix (src/updaters/anderson.jl:211-224) and history append.
 __global__ void k_anderson_mix(int h, double* w, const double* __restrict__ r, HistPtrs fh,
                                HistPtrs rh, const double* __restrict__ f_iter, MixCoef c,

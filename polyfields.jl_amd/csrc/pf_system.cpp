@@ -503,12 +503,32 @@ std::vector<double> System::scfstress(int exact) {
 // Anderson mixing (src/updaters/anderson.jl)
 
 Anderson::Anderson(int nh, int ie, double le) : nhist(nh), iter_euler(ie <= 1 ? 1 : ie), lam_euler(le) {
-    if (nhist < 1 || nhist > PF_MAXHIST) throw Error(-2, "nhist must be in 1..16");
+    if (nhist < 1) throw Error(-2, "nhist must be at least 1");
+}
+
+static void free_deep(Anderson& a) {
+    dev_free(a.gpart);
+    dev_free(a.gout);
+    dev_free(a.dcoef);
+    if (a.hgram) (void)hipHostFree(a.hgram);
+    if (a.hcoef) (void)hipHostFree(a.hcoef);
+    a.gpart = a.gout = a.dcoef = a.hgram = a.hcoef = nullptr;
 }
 
 Anderson::~Anderson() {
     dev_free(hist_r);
     dev_free(hist_w);
+    free_deep(*this);
+}
+
+HistRing Anderson::ring(int ih) const {
+    HistRing g;
+    g.r = hist_r;
+    g.w = hist_w;
+    g.N = sys->n * sys->nmon();
+    g.nslots = nhist + 1;
+    g.start = base + ih - 1;
+    return g;
 }
 
 double* Anderson::rslot(int logical) const {
@@ -534,6 +554,16 @@ void Anderson::setup(System* s) {
     PF_HIP(hipMemsetAsync(hist_w, 0, sizeof(double) * N * (nhist + 1), s->stream));
     base = 0;
     launch_copy(wslot(0), s->w, N, s->stream);
+    free_deep(*this);
+    if (nhist > PF_MAXHIST) {
+        const int nt = gram_tiles(nhist);
+        gparts = gram_tiled_parts(N);
+        gpart = (double*)dev_alloc(sizeof(double) * (size_t)gparts * nt * 64);
+        gout = (double*)dev_alloc(sizeof(double) * (size_t)nt * 64);
+        dcoef = (double*)dev_alloc(sizeof(double) * nhist);
+        PF_HIP(hipHostMalloc((void**)&hgram, sizeof(double) * (size_t)nt * 64, hipHostMallocDefault));
+        PF_HIP(hipHostMalloc((void**)&hcoef, sizeof(double) * nhist, hipHostMallocDefault));
+    }
     iter = 0;
     iter_hist = 0;
     lam = 0.0;
@@ -558,36 +588,59 @@ void Anderson::step() {
     if (euler) {
         launch_axpy(s.w, s.res, lam_euler, N, s.stream);
     } else {
-        HistPtrs hp{};
-        for (int m = 0; m < ih; ++m) hp.p[m] = rslot(ih - (m + 1));
-        const int nu = ih * (ih + 1) / 2;
-        launch_gram(ih, s.res, hp, N, s.partial, reduce_parts(N), s.sums + 64, s.stream);
-        PF_HIP(hipMemcpyAsync(s.pinned, s.sums + 64, sizeof(double) * (nu + ih),
-                              hipMemcpyDeviceToHost, s.stream));
-        s.sync();
+        const bool deep = nhist > PF_MAXHIST;
         lastU.assign(ih * ih, 0.0);
         lastV.assign(ih, 0.0);
-        int k = 0;
-        for (int m = 0; m < ih; ++m)
-            for (int nn = m; nn < ih; ++nn) {
-                lastU[m * ih + nn] = lastU[nn * ih + m] = s.pinned[k++];
+        if (!deep) {
+            HistPtrs hp{};
+            for (int m = 0; m < ih; ++m) hp.p[m] = rslot(ih - (m + 1));
+            const int nu = ih * (ih + 1) / 2;
+            launch_gram(ih, s.res, hp, N, s.partial, reduce_parts(N), s.sums + 64, s.stream);
+            PF_HIP(hipMemcpyAsync(s.pinned, s.sums + 64, sizeof(double) * (nu + ih),
+                                  hipMemcpyDeviceToHost, s.stream));
+            s.sync();
+            int k = 0;
+            for (int m = 0; m < ih; ++m)
+                for (int nn = m; nn < ih; ++nn) {
+                    lastU[m * ih + nn] = lastU[nn * ih + m] = s.pinned[k++];
+                }
+            for (int m = 0; m < ih; ++m) lastV[m] = s.pinned[nu + m];
+        } else {
+            const HistRing g = ring(ih);
+            launch_gram_tiled(ih, s.res, g, gpart, gout, s.stream);
+            PF_HIP(hipMemcpyAsync(hgram, gout, sizeof(double) * (size_t)gram_tiles(ih) * 64,
+                                  hipMemcpyDeviceToHost, s.stream));
+            s.sync();
+            for (int m = 0; m < ih; ++m) {
+                for (int nn = m; nn < ih; ++nn)
+                    lastU[m * ih + nn] = lastU[nn * ih + m] = hgram[gram_tile_index(m, nn)];
+                lastV[m] = hgram[gram_tile_index(m, ih)];
             }
-        for (int m = 0; m < ih; ++m) lastV[m] = s.pinned[nu + m];
+        }
         std::vector<double> inv(ih * ih);
         if (!lu_inverse(ih, lastU.data(), inv.data()))
             throw Error(-4, "Error inverting Umn matrix on iteration " + std::to_string(iter) +
                                 ": singular matrix");
         lastC.assign(ih, 0.0);
         matvec(ih, inv.data(), lastV.data(), lastC.data());
-        HistPtrs fh{}, rh{};
-        MixCoef c{};
-        for (int nn = 0; nn < ih; ++nn) {
-            fh.p[nn] = wslot(ih - (nn + 1));
-            rh.p[nn] = rslot(ih - (nn + 1));
-            c.C[nn] = lastC[nn];
-        }
         double* f_out = (iter < nhist + 1) ? wslot(iter) : wslot(0);
-        launch_anderson_mix(ih, s.w, s.res, fh, rh, wslot(ih), c, lam, N, f_out, s.stream);
+        if (!deep) {
+            HistPtrs fh{}, rh{};
+            MixCoef c{};
+            for (int nn = 0; nn < ih; ++nn) {
+                fh.p[nn] = wslot(ih - (nn + 1));
+                rh.p[nn] = rslot(ih - (nn + 1));
+                c.C[nn] = lastC[nn];
+            }
+            launch_anderson_mix(ih, s.w, s.res, fh, rh, wslot(ih), c, lam, N, f_out, s.stream);
+        } else {
+            // the previous step's copy of hcoef finished before the Gram read above
+            for (int nn = 0; nn < ih; ++nn) hcoef[nn] = lastC[nn];
+            PF_HIP(hipMemcpyAsync(dcoef, hcoef, sizeof(double) * ih, hipMemcpyHostToDevice,
+                                  s.stream));
+            launch_anderson_mix_ring(ih, s.w, s.res, ring(ih), wslot(ih), dcoef, lam, f_out,
+                                     s.stream);
+        }
     }
     if (iter < nhist + 1) {
         if (euler) launch_copy(wslot(iter), s.w, N, s.stream);
@@ -730,7 +783,8 @@ static bool isconverged(double ferr, double ftol, double serr, double stol, bool
     return f && sc;
 }
 
-ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
+ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o,
+                 const std::function<void(int)>& save) {
     // src/simulate.jl:60-148
     const auto t0 = std::chrono::steady_clock::now();
     s.validate();
@@ -769,8 +823,11 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
                 if (o.nsout > 0) std::printf("Updating cell dimensions: serr = %.5e\n", serr);
             }
         }
+        // a save writes the fields as the update left them, so no residuals! (which
+        // mean-subtracts the fields) is queued ahead of it
+        const bool save_due = save && o.nsave > 0 && step % o.nsave == 0;
         s.errors_launch();
-        if (queue && step + 1 < o.nsteps) {
+        if (queue && !save_due && step + 1 < o.nsteps) {
             s.residuals();
             s.res_queued = true;
         }
@@ -778,11 +835,13 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o) {
         if (std::isnan(ferr)) throw Error(-5, "Diverging field updater -- breaking.");
         conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
         if (conv) break;
+        if (save_due) save(step);
         step += 1;
     }
     s.residuals();
     s.errors(&ferr, &fhelm);
     if (vcell) serr = d->stress_error();
+    if (save) save(-1);
     ScftResults r;
     r.time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     r.nsteps = step;

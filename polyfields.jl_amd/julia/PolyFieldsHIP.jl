@@ -112,7 +112,7 @@ function add_interaction!(sys::HIPSystem, itx::EdwardsInteraction)
     return nothing
 end
 
-function setfield!(sys::HIPSystem, mid::Integer, w::Array{Float64,3})
+function set_omega!(sys::HIPSystem, mid::Integer, w::Array{Float64,3})
     @assert size(w) == sys.dims
     check(ccall((:pf_system_set_field, LIB), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
                 sys.handle, mid, w))
@@ -141,7 +141,7 @@ function fieldinit!(sys::HIPSystem; seed::Integer = -1, scale::Real = 0.1)
         order[mid] = nothing
     end
     for mid in keys(order)
-        setfield!(sys, mid, scale * randn(sys.dims))
+        set_omega!(sys, mid, scale * randn(sys.dims))
     end
     residuals!(sys)
 end
@@ -306,7 +306,7 @@ The free energy as a function of the fields and the cell parameters: writes them
 """
 function fhelm(sys::HIPSystem, w::Dict, cellparams::AbstractVector; functional::Symbol = :reference)
     for (mid, grid) in w
-        setfield!(sys, mid, Array{Float64,3}(grid))
+        set_omega!(sys, mid, Array{Float64,3}(grid))
     end
     p = Vector{Float64}(cellparams)
     check(ccall((:pf_system_cell, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint),
@@ -321,7 +321,7 @@ function __init__()
         @eval function $(CRC).rrule(::typeof(fhelm), sys::HIPSystem, w::Dict,
                                     cellparams::AbstractVector; functional::Symbol = :reference)
             for (mid, grid) in w
-                setfield!(sys, mid, Array{Float64,3}(grid))
+                set_omega!(sys, mid, Array{Float64,3}(grid))
             end
             p = Vector{Float64}(cellparams)
             check(ccall((:pf_system_cell, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint),

@@ -29,6 +29,9 @@ class ScftResults(C.Structure):
                 ("fhelm", D)]
 
 
+SaveFn = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
+
+
 SIGNATURES = {
     "pf_last_error": (C.c_char_p, []),
     "pf_version": (I, []),
@@ -52,6 +55,7 @@ SIGNATURES = {
     "pf_system_get_propagator": (I, [P, I, I, I, DP]),
     "pf_system_species_scalars": (I, [P, I, DP, DP, DP]),
     "pf_system_cell": (I, [P, DP, I, I]),
+    "pf_system_add_cell": (I, [P, I, S, DP, I]),
     "pf_system_synchronize": (I, [P]),
     "pf_system_set_precision": (I, [P, I]),
     "pf_system_set_mde": (I, [P, I]),
@@ -79,6 +83,7 @@ SIGNATURES = {
     "pf_domain_stress": (I, [P, I, DP, I, DP]),
     "pf_domain_destroy": (I, [P]),
     "pf_scft": (I, [P, P, P, C.POINTER(ScftOptions), C.POINTER(ScftResults)]),
+    "pf_scft_save": (I, [P, P, P, C.POINTER(ScftOptions), C.POINTER(ScftResults), SaveFn, P]),
     "pf_batch_create": (I, [PP, C.POINTER(C.c_void_p), I]),
     "pf_batch_destroy": (I, [P]),
     "pf_batch_density": (I, [P]),

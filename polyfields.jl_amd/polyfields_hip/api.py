@@ -19,7 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import ScftOptions as _CO, ScftResults as _CR, call, dptr, iptr
+from ._lib import SaveFn as _SaveFn, ScftOptions as _CO, ScftResults as _CR, call, dptr, iptr
 
 CANONICAL, GRAND = "canonical", "grand"
 _GRID = {"field": 0, "density": 1, "residual": 2, "potential": 3}
@@ -357,6 +357,17 @@ class FieldSystem:
     def potentials(self):
         return _GridView(self, "potential")
 
+    def add_cell(self, cell):
+        """``add_cell!(sys, cell)`` (src/system/system.jl:102-106): the system takes
+        ``cell`` (any crystal system of the grid's dimension) in place of its own."""
+        p = np.array(cell.params, dtype=float)
+        call("pf_system_add_cell", self._h, int(cell.dim), cell.crystal.encode(), dptr(p),
+             len(p))
+        if self.cell is not None and self.cell is not cell:
+            self.cell._sys = None
+        self.cell = cell
+        cell._sys = self
+
     def add_monomer(self, mon):
         call("pf_system_add_monomer", self._h, mon.id, mon.vol, mon.charge)
         self.monomers[mon.id] = mon
@@ -612,12 +623,33 @@ class SCFTResults:
 
 def scft(sys, updater, domain=None, opts=None):
     """``scft!(sys, updater; domain, opts)`` (src/simulate.jl:60-148); the loop runs in
-    the library."""
+    the library.  With ``opts.savepath`` set, the fields are written to it (JSON,
+    ``savefields``) every ``opts.nsave`` steps as the reference does
+    (src/simulate.jl:124-128), and once more after the final ``residuals!``: the
+    reference's own final write names an option that ``SCFTOptions`` lacks
+    (src/simulate.jl:140-142), so that write is the evident intent, not a copy."""
     opts = opts or SCFTOptions()
     o = _CO(opts.nsteps, opts.nsout, opts.nsave, opts.ftol, opts.stol)
     r = _CR()
-    call("pf_scft", sys._h, updater._h, domain._h if domain is not None else None, C.byref(o),
-         C.byref(r))
+    dh = domain._h if domain is not None else None
+    if opts.savepath:
+        from . import fileio
+        failure = []
+
+        def save(_user, _step):
+            if failure:
+                return
+            try:
+                fileio.savefields(sys, opts.savepath)
+            except BaseException as err:   # raised again below, after the loop returns
+                failure.append(err)
+
+        cb = _SaveFn(save)
+        call("pf_scft_save", sys._h, updater._h, dh, C.byref(o), C.byref(r), cb, None)
+        if failure:
+            raise failure[0]
+    else:
+        call("pf_scft", sys._h, updater._h, dh, C.byref(o), C.byref(r))
     updater.system = sys
     if domain is not None:
         domain.system = sys

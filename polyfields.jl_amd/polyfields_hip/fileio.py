@@ -149,7 +149,6 @@ def fieldinit_file(sys, path, load_cell=True, interpolate=True):
     """``fieldinit!(sys, fpath; load_cell, interpolate)`` (src/fileio.jl:56-64)."""
     d = loadfields(path)
     if load_cell:
-        if d["crystal"] != sys.cell.crystal or d["dim"] != sys.cell.dim:
-            raise ValueError("the file's crystal system differs from the system's cell")
-        sys.cell.params = d["cell_params"]
+        from .api import UnitCell
+        sys.add_cell(UnitCell(d["dim"], d["crystal"], *d["cell_params"]))
     fieldinit_dict(sys, d["fields"], field_type=d["field_type"], interpolate=interpolate)

@@ -1,0 +1,89 @@
+"""The 64^3 cubic variable-cell workload (tests/config2.py) on the GPU.
+
+* 64^3, FP64, against the oracle: the first steps of scft! (momentum steps and two cell
+  updates of two Newton iterations each) agree to rounding.
+* 16^3 (same cell, coarser grid), FP64, against the oracle over 600 steps with six cell
+  updates of ten Newton iterations: this trajectory moves by about 2e-10 relative under
+  one-ulp perturbations of the oracle's transforms, so the bound is 1e-8.
+* 64^3 to convergence, FP64 and FP32: the FP64 run converges (ferr, serr < 1e-5); the
+  FP32 run's error against it is reported, with bounds set by the FP32 residual floor
+  (the FP32 propagator moves the densities by about 1e-5 relative), and at the FP64
+  fixed point the FP32 propagator's fhelm and densities are within 1e-6 / 2e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from config2 import amplitudes, build, run
+
+pytestmark = pytest.mark.gpu
+
+P = pytest.importorskip("polyfields_hip")
+
+
+def _compare(sg, cg, so, co):
+    d = {"fhelm": abs(sg.free_energy() - so.free_energy()) / abs(so.free_energy()),
+         "a": abs(cg.params[0] - co.params[0]) / co.params[0]}
+    for m in (1, 2):
+        a, b = np.asarray(sg.density[m]), np.asarray(so.density[m])
+        d[f"rho{m}"] = float(np.abs(a - b).max())
+    return d
+
+
+def test_config2_64_first_steps_fp64():
+    sg, cg, _ = build(P, precision="fp64")
+    so, co, _ = build(O)
+    O.set_fft_workers(8)
+    try:
+        rg = run(P, sg, nsteps=6, nskip=5, nper=2)
+        ro = run(O, so, nsteps=6, nskip=5, nper=2)
+    finally:
+        O.set_fft_workers(1)
+    assert rg.nsteps == ro.nsteps == 6
+    d = _compare(sg, cg, so, co)
+    print("64^3 first steps", {k: f"{v:.2e}" for k, v in d.items()}, cg.params[0])
+    assert cg.params[0] != pytest.approx(16.329931618554522, rel=1e-6)   # the cell moved
+    assert d["fhelm"] < 1e-10 and d["a"] < 1e-10
+    assert d["rho1"] < 1e-10 and d["rho2"] < 1e-10
+    assert rg.ferr == pytest.approx(ro.ferr, rel=1e-9)
+    assert rg.serr == pytest.approx(ro.serr, rel=1e-7)
+
+
+def test_config2_16_trajectory_fp64():
+    sg, cg, _ = build(P, grid=16, precision="fp64")
+    so, co, _ = build(O, grid=16)
+    rg = run(P, sg, nsteps=600)
+    ro = run(O, so, nsteps=600)
+    assert rg.nsteps == ro.nsteps
+    d = _compare(sg, cg, so, co)
+    print("16^3 600 steps", {k: f"{v:.2e}" for k, v in d.items()}, cg.params[0], rg.ferr)
+    assert cg.params[0] > 18.0                       # the cell grew from 16.33
+    assert d["fhelm"] < 1e-8 and d["a"] < 1e-8
+    assert d["rho1"] < 1e-7 and d["rho2"] < 1e-7
+    assert rg.ferr == pytest.approx(ro.ferr, rel=1e-6)
+
+
+def test_config2_64_converged_fp64_and_fp32():
+    s64, c64, _ = build(P, precision="fp64")
+    r64 = run(P, s64, nsteps=5000)
+    assert r64.converged and r64.ferr < 1e-5 and r64.serr <= 1e-5
+    f64, a64 = s64.free_energy(), float(c64.params[0])
+    rho64 = {m: np.array(s64.density[m]) for m in (1, 2)}
+    # the FP32 propagator at the FP64 fixed point (same fields, same cell)
+    s64.set_precision("fp32")
+    s64.density_update()
+    f32_at = s64.free_energy()
+    d_at = max(float(np.abs(np.asarray(s64.density[m]) - rho64[m]).max()) for m in (1, 2))
+    # an FP32 run from the seed; its residual floor sits near 1e-4, so it stops there
+    s32, c32, _ = build(P, precision="fp32")
+    r32 = run(P, s32, nsteps=5000, ftol=2e-4)
+    a32 = float(c32.params[0])
+    amp = max(float(np.abs(amplitudes(s32.density[m]) - amplitudes(rho64[m])).max() /
+                    amplitudes(rho64[m]).max()) for m in (1, 2))
+    print("64^3 converged: fp64 steps", r64.nsteps, "fhelm", f64, "a", a64,
+          "| fp32 at the fp64 state: dfhelm %.2e drho %.2e" % (abs(f32_at - f64), d_at),
+          "| fp32 run: steps", r32.nsteps, "ferr %.2e" % r32.ferr, "dfhelm %.2e" %
+          abs(r32.fhelm - f64), "da/a %.2e" % (abs(a32 - a64) / a64), "damp %.2e" % amp)
+    assert abs(f32_at - f64) < 1e-6 and d_at < 2e-5
+    assert r32.converged
+    assert abs(r32.fhelm - f64) < 1e-4 and abs(a32 - a64) / a64 < 1e-2 and amp < 1e-2
