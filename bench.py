@@ -13,9 +13,12 @@ Secondary (`batched_scft`): SCFT solves/s of the batched sweep workload -- 2-D h
 diblocks with grids drawn from {32^2, 48^2, 64^2, 96^2}, Ns in [64, 128], integer N_A in
 [25, 45], chi N in [12, 30], a in [3, 6] Rg, fields 0.1*N(0,1) -- batched in FP32 versus
 the plain FP64 path solving the same instances one at a time.  Protocol for both paths:
-scft!(AndersonUpdater(nhist=10, iter_euler=10^9, lam_euler=0.1); ftol=1e-2) (pure Euler
-mixing) then scft!(AndersonUpdater(nhist=10); ftol=1e-6); a solve is an instance reaching
-ferr < 1e-6.  The batched path runs the Anderson phase in FP32 to ferr < 1e-5, then
+scft!(AndersonUpdater(nhist=10, iter_euler=10^9, lam_euler=0.05); nsteps=1500, ftol=1e-2)
+(pure Euler mixing) then scft!(AndersonUpdater(nhist=20); ftol=1e-6); a solve is an
+instance reaching ferr < 1e-6.  (Euler steps of 0.1 oscillate without converging on about
+two thirds of these seeds, in the oracle as on the GPU, and Anderson histories of 5 or 10
+diverge on most of the rest: the reference's mixing step reaches 1 once the history is
+full.)  The batched path runs the Anderson phase in FP32 to ferr < 1e-5, then
 switches the propagator to FP64 and finishes to 1e-6 (FP32 residual noise sits near
 1e-6).
 """
@@ -124,8 +127,8 @@ def sweep_system(P, c, precision):
     return s
 
 
-PH1 = dict(nhist=10, iter_euler=10 ** 9, lam_euler=0.1, nsteps=1000, ftol=1e-2)
-PH2 = dict(nhist=10, iter_euler=1, lam_euler=0.01, nsteps=1000, ftol=1e-6)
+PH1 = dict(nhist=10, iter_euler=10 ** 9, lam_euler=0.05, nsteps=1500, ftol=1e-2)
+PH2 = dict(nhist=20, iter_euler=1, lam_euler=0.01, nsteps=1000, ftol=1e-6)
 PH2_FP32 = dict(PH2, ftol=1e-5)
 
 
@@ -161,10 +164,10 @@ def batched_scft(P, count, plain_sample):
         t0 = time.perf_counter()
         ok = False
         try:
-            q1 = P.scft(s, P.AndersonUpdater(nhist=10, iter_euler=10 ** 9, lam_euler=0.1),
+            q1 = P.scft(s, P.AndersonUpdater(nhist=10, iter_euler=10 ** 9, lam_euler=0.05),
                         opts=P.SCFTOptions(nsteps=PH1["nsteps"], ftol=PH1["ftol"]))
             if q1.converged:
-                q2 = P.scft(s, P.AndersonUpdater(nhist=10),
+                q2 = P.scft(s, P.AndersonUpdater(nhist=20),
                             opts=P.SCFTOptions(nsteps=PH2["nsteps"], ftol=PH2["ftol"]))
                 ok = q2.converged
         except P.PolyFieldsError:
@@ -189,8 +192,8 @@ def batched_scft(P, count, plain_sample):
         "fp64_batched_solved": n64,
         "fhelm_abs_err_vs_plain_sample": {"max": max(errs) if errs else None,
                                           "n": len(errs)},
-        "protocol": "scft!(Anderson iter_euler=1e9 lam_euler=0.1, ftol 1e-2) then "
-                    "scft!(Anderson nhist=10, ftol 1e-6); the batched path runs the second "
+        "protocol": "scft!(Anderson iter_euler=1e9 lam_euler=0.05, ftol 1e-2) then "
+                    "scft!(Anderson nhist=20, ftol 1e-6); the batched path runs the second "
                     "phase in FP32 to ftol 1e-5 and polishes in FP64 to 1e-6; solved = "
                     "ferr < 1e-6 in FP64",
     }
@@ -369,7 +372,7 @@ def grad_workload(P, args):
 SCFT64 = dict(lam=0.1, nskip=100, nper=10, ftol=1e-5, stol=1e-5, nsteps=5000)
 
 
-def scft64_run(P, precision, seed, ftol):
+def scft64_run(P, precision, seed, ftol, stol=SCFT64["stol"]):
     s, ch = build(P, precision)
     for m, w in seeded_fields(s.shape, seed).items():
         s.fields[m] = w
@@ -378,14 +381,15 @@ def scft64_run(P, precision, seed, ftol):
     t0 = time.perf_counter()
     r = P.scft(s, P.MomentumUpdater(method="MOM", lam=SCFT64["lam"]),
                domain=P.DomainUpdater(nskip=SCFT64["nskip"], nper=SCFT64["nper"]),
-               opts=P.SCFTOptions(nsteps=SCFT64["nsteps"], ftol=ftol, stol=SCFT64["stol"]))
+               opts=P.SCFTOptions(nsteps=SCFT64["nsteps"], ftol=ftol, stol=stol))
     s.synchronize()
     return s, r, time.perf_counter() - t0
 
 
 def scft64_workload(P, args):
     """The 64^3 variable-cell SCFT from the BCC-biased seed, to convergence, in the FP64
-    plain mode and with the FP32 propagator (stopped at its residual floor, ftol 2e-4),
+    plain mode and with the FP32 propagator (stopped at its residual and stress floors,
+    ftol 2e-4, stol 3e-5),
     with the FP32 run's and the FP32 propagator's errors against FP64."""
     s64, r64, t64 = scft64_run(P, "fp64", args.seed, SCFT64["ftol"])
     a64, f64 = float(s64.cell.params[0]), s64.free_energy()
@@ -396,7 +400,7 @@ def scft64_workload(P, args):
           "rho_max_abs": max(float(np.abs(np.asarray(s64.density[m]) - rho64[m]).max())
                              for m in s64.mids)}
     del s64
-    s32, r32, t32 = scft64_run(P, "fp32", args.seed, 2e-4)
+    s32, r32, t32 = scft64_run(P, "fp32", args.seed, 2e-4, 3e-5)
     a32 = float(s32.cell.params[0])
     amp = lambda r: np.abs(np.fft.rfftn(np.asarray(r)))
     err = {"fhelm_abs": abs(r32.fhelm - r64.fhelm), "a_rel": abs(a32 - a64) / a64,
@@ -404,6 +408,22 @@ def scft64_workload(P, args):
                               for m in s32.mids),
            "rho_amp_rel": max(float(np.abs(amp(s32.density[m]) - amp(rho64[m])).max() /
                                     amp(rho64[m]).max()) for m in s32.mids)}
+    # FP64 polish of the FP32 run: the same updaters continue with the FP64 propagator to
+    # the FP64 run's tolerances
+    s32.set_precision("fp64")
+    t0 = time.perf_counter()
+    rp = P.scft(s32, P.MomentumUpdater(method="MOM", lam=SCFT64["lam"]),
+                domain=P.DomainUpdater(nskip=SCFT64["nskip"], nper=SCFT64["nper"]),
+                opts=P.SCFTOptions(nsteps=SCFT64["nsteps"], ftol=SCFT64["ftol"],
+                                   stol=SCFT64["stol"]))
+    s32.synchronize()
+    tp = time.perf_counter() - t0
+    ap = float(s32.cell.params[0])
+    pol = {"seconds": tp, "nsteps": rp.nsteps, "converged": rp.converged,
+           "fhelm_abs_err_vs_fp64": abs(rp.fhelm - r64.fhelm), "a_rel_err_vs_fp64":
+           abs(ap - a64) / a64,
+           "rho_max_abs_err_vs_fp64": max(float(np.abs(np.asarray(s32.density[m]) -
+                                                       rho64[m]).max()) for m in s32.mids)}
     res = lambda r, t, a: {"seconds": t, "nsteps": r.nsteps, "converged": r.converged,
                            "ferr": r.ferr, "serr": r.serr, "fhelm": r.fhelm, "a": a}
     return {"metric": "64^3 variable-cell SCFT solves/s (FP64 plain mode)", "value": 1.0 / t64,
@@ -415,7 +435,8 @@ def scft64_workload(P, args):
                                    "a0=4 Rg; scft! with MomentumUpdater(MOM, lam=0.1) + "
                                    "DomainUpdater(nskip=100, nper=10), ftol=stol=1e-5"},
             "fp64": res(r64, t64, a64), "fp32": res(r32, t32, a32),
-            "fp32_run_error_vs_fp64": err, "fp32_propagator_at_fp64_state": at}
+            "fp32_run_error_vs_fp64": err, "fp32_propagator_at_fp64_state": at,
+            "fp32_then_fp64_polish": pol}
 
 
 def main():

@@ -37,6 +37,22 @@ def irfft3(ak, shape):
 CANONICAL, GRAND = "canonical", "grand"
 
 
+def _julia_log(x):
+    """Julia's ``log`` of a Float64: NaN for NaN, -Inf at 0, +Inf at +Inf, DomainError
+    (raised here as ValueError) below 0.  A diverging field state (Q overflowing to Inf
+    or turning NaN) then runs on to the NaN field error that scft! reports, as in the
+    reference, instead of stopping at Python's ``math.log`` domain check."""
+    x = float(x)
+    if x != x or x == math.inf:
+        return x
+    if x == 0.0:
+        return -math.inf
+    if x < 0.0:
+        raise ValueError(f"DomainError with {x}: log will only return a complex result "
+                         "if called with a complex argument")
+    return math.log(x)
+
+
 class Compressibility:
     """src/system/compress.jl:13-61."""
 
@@ -372,7 +388,7 @@ class FieldSystem:
         """src/system/system.jl:248-259."""
         for isp, sp in enumerate(self.species):
             if self.ensemble == CANONICAL:
-                self.mu_species[isp] = math.log(self.phi_species[isp] / sp.Q)
+                self.mu_species[isp] = _julia_log(self.phi_species[isp] / sp.Q)
             else:
                 self.phi_species[isp] = sp.Q * math.exp(self.mu_species[isp])
 

@@ -136,9 +136,17 @@ __global__ void k_final_b(const SysTask* __restrict__ tasks, int nv, int off, in
     const SysTask* t = tasks + blockIdx.x;
     const double* part = use_gram ? t->gpartial : t->partial;
     const int np = use_gram ? t->gparts : t->nparts;
+    // the system sums (off 32) carry the muphi! domain failure of one system as a NaN
+    // field-error sum: the batch drops it as diverged (log(phi / Q) of a negative ratio
+    // raises in the reference, src/system/system.jl:251)
+    bool bad = false;
+    if (off == 32 && t->ra.canonical)
+        for (int i = 0; i < t->nsp; ++i)
+            if (t->sc->phi[i] / t->sc->Q[i] < 0.0) bad = true;
     for (int k = threadIdx.x; k < nv; k += blockDim.x) {
         double s = 0.0;
         for (int p = 0; p < np; ++p) s += part[(size_t)p * nv + k];
+        if (bad && k == 0) s = __builtin_nan("");
         t->sums[off + k] = s;
         if (compact) compact[(size_t)blockIdx.x * nv + k] = s;
     }

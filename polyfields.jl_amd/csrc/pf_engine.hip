@@ -565,32 +565,32 @@ class Engine : public EngineBase {
             launch_point_density(s.w + (size_t)slot * n, s.mon_vol[slot], n, sp.rho_sp,
                                  s.partial, nparts, s.sc, (int)i, st);
         }
-        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), n, st);
+        if (!tasks.empty())
+            launch_simpson<T>(d_tasks, (int)tasks.size(), n, st, n % (16 / sizeof(T)) == 0);
     }
 
-    void build_stress() {
+    // Stress scratch (src/species/multiblock.jl:233-295): the row table, the per-row sums,
+    // and for 3-D grids the k-space partial sums and the transforms of q_s and q+_s for
+    // `chunk` rows at a time.
+    struct StressBufs {
+        StressLane<T>* d_slanes = nullptr;
+        double* srows = nullptr;
+        double* spart = nullptr;
+        cpx<T>* kstress = nullptr;
+        int chunk = 1;
+    };
+    bool stress_fused() const { return planar && 2 * lds_plane <= 160 * 1024; }
+    size_t spart_len() const {
+        return (size_t)std::max(nrows, 1) * S->dims[1] * ntiles * PF_MAXPAR;
+    }
+
+    // the row table for buffers `bf` (host copy in `out`)
+    void fill_stress_lanes(std::vector<StressLane<T>>& out, const StressBufs& bf) {
         System& s = *S;
         const int nx = s.dims[0], ny = s.dims[1], nz = s.dims[2];
         const size_t n = s.n;
-        const bool fused = planar && 2 * lds_plane <= 160 * 1024;
-        if (slanes.empty() || (int)slanes.size() != nrows) {
-            dev_free(d_slanes);
-            dev_free(kstress);
-            dev_free(srows);
-            dev_free(spart);
-            d_slanes = nullptr;
-            kstress = nullptr;
-            slanes.assign(nrows, StressLane<T>{});
-            if (nrows) d_slanes = (StressLane<T>*)dev_alloc(sizeof(StressLane<T>) * nrows);
-            srows = (double*)dev_alloc(sizeof(double) * std::max(nrows, 1) * PF_MAXPAR);
-            if (!fused) {
-                schunk = (int)std::max<size_t>(1, std::min<size_t>(nrows, (size_t)(256u << 20) /
-                                                                             (2 * nk * sizeof(cpx<T>))));
-                kstress = (cpx<T>*)dev_alloc(sizeof(cpx<T>) * 2 * nk * schunk);
-                spart = (double*)dev_alloc(sizeof(double) * (size_t)std::max(nrows, 1) * ny *
-                                           ntiles * PF_MAXPAR);
-            }
-        }
+        const bool fused = stress_fused();
+        out.assign(nrows, StressLane<T>{});
         const int np = (int)s.cell.params.size();
         for (size_t i = 0; i < s.species.size(); ++i) {
             const Species& sp = s.species[i];
@@ -598,7 +598,7 @@ class Engine : public EngineBase {
             const SpBuf& b = sb[i];
             for (int sl = 1; sl <= sp.Ns + 1; ++sl) {
                 const int r = b.row0 + sl - 1;
-                StressLane<T>& L = slanes[r];
+                StressLane<T>& L = out[r];
                 L.nx = nx;
                 L.ny = ny;
                 L.nz = nz;
@@ -624,14 +624,46 @@ class Engine : public EngineBase {
                 L.py = py;
                 L.pz = pz;
                 if (!fused) {
-                    const int c = r % schunk;
-                    L.kq = kstress + (size_t)(2 * c) * nk;
-                    L.kqc = kstress + (size_t)(2 * c + 1) * nk;
+                    const int c = r % bf.chunk;
+                    L.kq = bf.kstress + (size_t)(2 * c) * nk;
+                    L.kqc = bf.kstress + (size_t)(2 * c + 1) * nk;
                 }
             }
         }
+    }
+
+    void build_stress() {
+        const bool fused = stress_fused();
+        if (slanes.empty() || (int)slanes.size() != nrows) {
+            dev_free(d_slanes);
+            dev_free(kstress);
+            dev_free(srows);
+            dev_free(spart);
+            d_slanes = nullptr;
+            kstress = nullptr;
+            spart = nullptr;
+            if (nrows) d_slanes = (StressLane<T>*)dev_alloc(sizeof(StressLane<T>) * nrows);
+            srows = (double*)dev_alloc(sizeof(double) * std::max(nrows, 1) * PF_MAXPAR);
+            if (!fused) {
+                schunk = (int)std::max<size_t>(1, std::min<size_t>(nrows, (size_t)(256u << 20) /
+                                                                             (2 * nk * sizeof(cpx<T>))));
+                kstress = (cpx<T>*)dev_alloc(sizeof(cpx<T>) * 2 * nk * schunk);
+                spart = (double*)dev_alloc(sizeof(double) * spart_len());
+            }
+        }
+        fill_stress_lanes(slanes, own_stress_bufs());
         if (nrows) upload(d_slanes, slanes.data(), sizeof(StressLane<T>) * nrows);
-        slanes_cell_version = s.cell.version;
+        slanes_cell_version = S->cell.version;
+    }
+
+    StressBufs own_stress_bufs() const {
+        StressBufs bf;
+        bf.d_slanes = d_slanes;
+        bf.srows = srows;
+        bf.spart = spart;
+        bf.kstress = kstress;
+        bf.chunk = std::max(schunk, 1);
+        return bf;
     }
 
     // Segment of a checkpointed chain whose stress rows include slice sl: bnd[m] .. bnd[m+1]-1,
@@ -643,24 +675,23 @@ class Engine : public EngineBase {
     }
 
     // stress rows [r0, r0 + cnt) into srows (planar) or the k-space partial sums (3-D)
-    void stress_launch(int r0, int cnt, hipStream_t st) {
+    void stress_launch(const StressBufs& bf, int r0, int cnt, hipStream_t st) {
         const System& s = *S;
-        if (planar && 2 * lds_plane <= 160 * 1024) {
-            launch_stress_planar<T>(d_slanes + r0, cnt, srows, 2 * lds_plane, 256, st);
+        if (stress_fused()) {
+            launch_stress_planar<T>(bf.d_slanes + r0, cnt, bf.srows, 2 * lds_plane, 256, st);
             return;
         }
         const int ny = s.dims[1], nz = s.dims[2];
-        for (int a = r0; a < r0 + cnt; a += schunk) {
-            const int c = std::min(schunk, r0 + cnt - a);
-            launch_stress_slab<T>(d_slanes + a, c, nz, lds_plane, 256, st);
-            launch_stress_pencil<T>(d_slanes + a, c, ny, ntiles, tile, spart, 2 * lds_pencil,
-                                    256, st);
+        for (int a = r0; a < r0 + cnt; a += bf.chunk) {
+            const int c = std::min(bf.chunk, r0 + cnt - a);
+            launch_stress_slab<T>(bf.d_slanes + a, c, nz, lds_plane, 256, st);
+            launch_stress_pencil<T>(bf.d_slanes + a, c, ny, ntiles, tile, bf.spart,
+                                    2 * lds_pencil, 256, st);
         }
     }
 
-    void stress_rows(System& s, std::vector<double>& rows) override {
-        if (s.dirty_species || sb.size() != s.species.size()) build();
-        if (slanes_cell_version != s.cell.version || (int)slanes.size() != nrows) build_stress();
+    // every stress row with buffers `bf`, copied to the host
+    void stress_run(System& s, const StressBufs& bf, std::vector<double>& rows) {
         rows.assign((size_t)nrows * PF_MAXPAR, 0.0);
         if (!nrows) return;
         hipStream_t st = s.stream;
@@ -675,18 +706,65 @@ class Engine : public EngineBase {
                     run_segment(b.jq, m, false);
                     run_segment(b.jc, m, false);
                     const int lo = b.bnd[m], hi = (m == nseg - 1) ? b.bnd[m + 1] : b.bnd[m + 1] - 1;
-                    stress_launch(b.row0 + lo - 1, hi - lo + 1, st);
+                    stress_launch(bf, b.row0 + lo - 1, hi - lo + 1, st);
                 }
             }
         } else {
-            stress_launch(0, nrows, st);
+            stress_launch(bf, 0, nrows, st);
         }
-        if (!(planar && 2 * lds_plane <= 160 * 1024))
-            launch_segsum(spart, nrows, s.dims[1] * ntiles, PF_MAXPAR, PF_MAXPAR, srows, st);
-        PF_HIP(hipMemcpyAsync(rows.data(), srows, sizeof(double) * nrows * PF_MAXPAR,
+        if (!stress_fused())
+            launch_segsum(bf.spart, nrows, s.dims[1] * ntiles, PF_MAXPAR, PF_MAXPAR, bf.srows, st);
+        PF_HIP(hipMemcpyAsync(rows.data(), bf.srows, sizeof(double) * nrows * PF_MAXPAR,
                               hipMemcpyDeviceToHost, st));
         PF_HIP(hipStreamSynchronize(st));
     }
+
+    void stress_rows(System& s, std::vector<double>& rows) override {
+        if (s.dirty_species || sb.size() != s.species.size()) build();
+        if (slanes_cell_version != s.cell.version || (int)slanes.size() != nrows) build_stress();
+        stress_run(s, own_stress_bufs(), rows);
+    }
+
+    // The same rows with every buffer carved from `scr` (cap bytes of device memory the
+    // caller owns): nothing is allocated on the device.  The propagator tables must exist
+    // already (a density! since the species last changed) and the stacks must be full.
+    void stress_rows_scratch(System& s, std::vector<double>& rows, char* scr,
+                             size_t cap) override {
+        if (s.dirty_species || sb.size() != s.species.size())
+            throw Error(-2, "the system has no propagator tables yet: run density! once "
+                            "before the allocation-free stress");
+        if (ckK)
+            throw Error(-2, "the allocation-free stress needs full q stacks, not checkpoints");
+        auto take = [&](size_t& off, size_t bytes) {
+            char* p = scr + off;
+            off += (bytes + 255) & ~(size_t)255;
+            return p;
+        };
+        size_t off = 0;
+        StressBufs bf;
+        bf.d_slanes = (StressLane<T>*)take(off, sizeof(StressLane<T>) * std::max(nrows, 1));
+        bf.srows = (double*)take(off, sizeof(double) * std::max(nrows, 1) * PF_MAXPAR);
+        size_t need = off;
+        if (!stress_fused()) {
+            bf.spart = (double*)take(off, sizeof(double) * spart_len());
+            const size_t per = 2 * nk * sizeof(cpx<T>);
+            need = off + per;
+            if (need <= cap)
+                bf.chunk = (int)std::max<size_t>(1, std::min<size_t>(nrows, (cap - off) / per));
+            bf.kstress = (cpx<T>*)(scr + off);
+        }
+        if (need > cap)
+            throw Error(-1, "the allocation-free stress needs " + std::to_string(need) +
+                                " bytes of scratch for this grid and contour; the library "
+                                "holds " + std::to_string(cap));
+        fill_stress_lanes(scratch_lanes, bf);
+        if (nrows)
+            PF_HIP(hipMemcpyAsync(bf.d_slanes, scratch_lanes.data(),
+                                  sizeof(StressLane<T>) * nrows, hipMemcpyHostToDevice,
+                                  s.stream));
+        stress_run(s, bf, rows);
+    }
+    std::vector<StressLane<T>> scratch_lanes;
 
     // Reverse-sweep launches: the compile-time kernels when the size has them (and the
     // schedule does not ask for the generic ones), else the generic ones.  apper records
@@ -767,7 +845,7 @@ class Engine : public EngineBase {
         for (auto& t : tk) t.rho_blk = dst[t.isp];
         auto* d = (SimpsonTask<T>*)dev_alloc(sizeof(SimpsonTask<T>) * tk.size());
         upload(d, tk.data(), sizeof(SimpsonTask<T>) * tk.size());
-        launch_simpson<T>(d, (int)tk.size(), s.n, s.stream);
+        launch_simpson<T>(d, (int)tk.size(), s.n, s.stream, s.n % (16 / sizeof(T)) == 0);
         PF_HIP(hipStreamSynchronize(s.stream));
         dev_free(d);
     }
@@ -1202,7 +1280,9 @@ class BatchEngine : public BatchEngineBase {
         const int nparts = reduce_parts(nmax);
         if (!tasks.empty()) {
             launch_species_Q<T>(d_tasks, (int)tasks.size(), nmax, partial, nparts, st);
-            launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
+            bool vec = true;
+            for (const auto& tk : tasks) vec = vec && tk.n % (16 / sizeof(T)) == 0;
+            launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st, vec);
         }
     }
 };

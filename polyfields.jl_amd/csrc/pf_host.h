@@ -127,6 +127,10 @@ struct System {
     double free_energy_of(const double v[5]);
     double free_energy_bulk();
     std::vector<double> scfstress(int exact = 0);
+    // scfstress with its device scratch in [scr, scr + cap) (no device allocation)
+    std::vector<double> scfstress_scratch(char* scr, size_t cap);
+    std::vector<double> stress_rows_combined(char* scr, size_t cap);
+    std::vector<double> stress_combine(const std::vector<double>& rows) const;
     void monomer_fractions(std::vector<double>& fr);
     void sync();
     void set_precision(int p);       // swap the propagator engine (fields stay FP64)
@@ -175,6 +179,9 @@ class EngineBase {
     virtual ~EngineBase() = default;
     virtual void density(System& s) = 0;
     virtual void stress_rows(System& s, std::vector<double>& rows) = 0;   // [row][PF_MAXPAR]
+    // the same rows with every device buffer carved from the caller's `cap` bytes at scr
+    virtual void stress_rows_scratch(System& s, std::vector<double>& rows, char* scr,
+                                     size_t cap) = 0;
     virtual void get_propagator(System& s, int isp, int slice, int backward, double* host) = 0;
     virtual void timing(int on) = 0;
     virtual double timing_ms(int which, int* count) = 0;

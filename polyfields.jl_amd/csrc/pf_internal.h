@@ -272,12 +272,16 @@ void launch_fill(T* p, T v, size_t n, hipStream_t st);
 template <typename T>
 void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, double* partial,
                       int nparts, hipStream_t st);
+// vec: every task's n is a multiple of 16 / sizeof(T) (16-byte slice loads)
 template <typename T>
-void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st);
+void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st,
+                    bool vec = false);
 void launch_point_density(const double* w, double vol, size_t n, double* rho_sp,
                           double* partial, int nparts, DevScalars* sc, int isp,
                           hipStream_t st);
 void launch_muphi(DevScalars* sc, int nsp, int canonical, hipStream_t st);
+void launch_domain_flag(const DevScalars* sc, int nsp, int canonical, double* out,
+                        hipStream_t st);
 void launch_pool(const PoolArgs& a, size_t n, DevScalars* sc, hipStream_t st);
 void launch_residuals(const ResidualArgs& a, size_t n, double* partial, int nparts,
                       double* sums, hipStream_t st);

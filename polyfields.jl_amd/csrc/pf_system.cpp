@@ -395,10 +395,24 @@ void System::uniform_fields() {
     fields_set = true;
 }
 
+// The five system sums at sums[32..36] and the muphi! domain flag at sums[37].
+static void launch_sums(System& s) {
+    launch_sys_reduce(s.residual_args(), s.n, s.partial, reduce_parts(s.n), s.sums + 32, s.stream);
+    launch_domain_flag(s.sc, (int)s.species.size(), s.canonical, s.sums + 37, s.stream);
+    PF_HIP(hipMemcpyAsync(s.pinned, s.sums + 32, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                          s.stream));
+}
+
+static void check_domain(const double* v) {
+    if (v[5] != 0.0)
+        throw Error(-5, "DomainError: log(phi / Q) of a negative partition-function ratio in "
+                        "muphi! (the propagator's Q turned negative: the fields diverged)");
+}
+
 void System::sys_sums(double out[5]) {
-    launch_sys_reduce(residual_args(), n, partial, reduce_parts(n), sums + 32, stream);
-    PF_HIP(hipMemcpyAsync(pinned, sums + 32, sizeof(double) * 5, hipMemcpyDeviceToHost, stream));
+    launch_sums(*this);
     sync();
+    check_domain(pinned);
     for (int k = 0; k < 5; ++k) out[k] = pinned[k];
 }
 
@@ -421,13 +435,13 @@ void System::errors(double* ferr, double* fhelm) {
 
 void System::errors_launch() {
     if (!err_ev) PF_HIP(hipEventCreateWithFlags(&err_ev, hipEventDisableTiming));
-    launch_sys_reduce(residual_args(), n, partial, reduce_parts(n), sums + 32, stream);
-    PF_HIP(hipMemcpyAsync(pinned, sums + 32, sizeof(double) * 5, hipMemcpyDeviceToHost, stream));
+    launch_sums(*this);
     PF_HIP(hipEventRecord(err_ev, stream));
 }
 
 void System::errors_read(double* ferr, double* fhelm) {
     PF_HIP(hipEventSynchronize(err_ev));
+    check_domain(pinned);
     double v[5];
     for (int k = 0; k < 5; ++k) v[k] = pinned[k];
     *ferr = std::sqrt(v[0]) / std::sqrt(EPS + v[1]);
@@ -464,15 +478,32 @@ double System::free_energy_bulk() {
     return f;
 }
 
-std::vector<double> System::scfstress(int exact) {
-    // src/system/system.jl:384-395 with src/species/multiblock.jl:233-295
-    if (exact) throw Error(-2, "only the reference stress formula is provided");
-    const int np = (int)cell.params.size();
-    std::vector<double> stress(np, 0.0);
+// src/system/system.jl:384-395 with src/species/multiblock.jl:233-295: the per-slice rows
+// of the k-space sums from the engine (its own buffers, or [scr, scr + cap) when scr is
+// given), then the Simpson-weighted combination per species on the host
+std::vector<double> System::stress_rows_combined(char* scr, size_t cap) {
+    std::vector<double> rows;
     launch_muphi(sc, (int)species.size(), canonical, stream);
     fetch_scalars();
-    std::vector<double> rows;
-    eng->stress_rows(*this, rows);
+    if (scr)
+        eng->stress_rows_scratch(*this, rows, scr, cap);
+    else
+        eng->stress_rows(*this, rows);
+    return stress_combine(rows);
+}
+
+std::vector<double> System::scfstress(int exact) {
+    if (exact) throw Error(-2, "only the reference stress formula is provided");
+    return stress_rows_combined(nullptr, 0);
+}
+
+std::vector<double> System::scfstress_scratch(char* scr, size_t cap) {
+    return stress_rows_combined(scr, cap);
+}
+
+std::vector<double> System::stress_combine(const std::vector<double>& rows) const {
+    const int np = (int)cell.params.size();
+    std::vector<double> stress(np, 0.0);
     const double n2 = (double)n * (double)n;
     int row0 = 0;
     for (size_t i = 0; i < species.size(); ++i) {

@@ -225,9 +225,20 @@ function scft!(sys::HIPSystem, updater::HIPUpdater; domain = nothing,
     o = Ref(PFOptions(opts.nsteps, opts.nsout, opts.nsave, opts.ftol, opts.stol))
     r = Ref(PFResults(0, 0, 0, 0, 0, 0))
     dh = domain === nothing ? C_NULL : domain.handle
-    check(ccall((:pf_scft, LIB), Cint,
-                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{PFOptions}, Ref{PFResults}),
-                sys.handle, updater.handle, dh, o, r))
+    if isempty(opts.savepath)
+        check(ccall((:pf_scft, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{PFOptions}, Ref{PFResults}),
+                    sys.handle, updater.handle, dh, o, r))
+    else
+        # savefields(sys, opts.savepath) every nsave steps (src/simulate.jl:124-128) and
+        # after the final residuals!: the library calls back at those points
+        job = (sys, opts.savepath)
+        cb = @cfunction(_save_cb, Cvoid, (Any, Cint))
+        check(ccall((:pf_scft_save, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{PFOptions}, Ref{PFResults},
+                     Ptr{Cvoid}, Any),
+                    sys.handle, updater.handle, dh, o, r, cb, job))
+    end
     x = r[]
     return SCFTResults(x.time, x.nsteps, x.converged != 0, x.ferr, x.serr, x.fhelm)
 end
@@ -289,6 +300,42 @@ function density_vjp(sys::HIPSystem, rho_bar::Dict, Q_bar::AbstractVector)
                 for (k, mid) in enumerate(sort(sys.mids))), dcell, dN[1:nb]
 end
 
+_save_cb(job, step::Cint) = (savefields(job[1], job[2]); nothing)
+
+"""
+    savefields(sys, fpath; field_type = :omega)
+
+The reference's JSON field file (src/fileio.jl:77-100) from the device state.
+"""
+function savefields(sys::HIPSystem, fpath::AbstractString; field_type::Symbol = :omega)
+    which = field_type == :rho ? 1 : 0
+    d = Dict(:dim => sys.cell.dim, :grid => sys.dims, :mids => sort(sys.mids),
+             :crystal => sys.cell.crystal, :cell_params => cellparams(sys),
+             :field_type => field_type,
+             :fields => Dict(mid => getgrid(sys, which, mid) for mid in sys.mids))
+    open(fpath, "w") do f
+        write(f, JSON.json(d, 4))
+    end
+    return nothing
+end
+
+function cellparams(sys::HIPSystem)
+    p = zeros(length(sys.cell.params))
+    check(ccall((:pf_system_cell, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Cint, Cint),
+                sys.handle, p, length(p), 0))
+    return p
+end
+
+"""`add_cell!(sys, cell)` (src/system/system.jl:102-106): the system takes `cell`."""
+function add_cell!(sys::HIPSystem, cell::UnitCell)
+    p = Vector{Float64}(cell.params)
+    check(ccall((:pf_system_add_cell, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Cstring, Ptr{Float64}, Cint),
+                sys.handle, cell.dim, String(cell.crystal), p, length(p)))
+    sys.cell = cell
+    return nothing
+end
+
 """
     set_checkpoint!(sys, K)
 
@@ -335,5 +382,6 @@ function __init__()
 end
 
 using Random
+import JSON
 
 end # module

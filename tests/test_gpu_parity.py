@@ -441,7 +441,8 @@ def test_fieldinit_from_reference_file_and_roundtrip(tmp_path):
     # the lam1d seed is a converged 64-point lamellar state of the chi = 0.25 diblock
     sg, cg, _ = diblock_system(P, (64, 1, 1), 1, "lamellar", (5.0,), chi=0.25)
     P.fieldinit(sg, path)
-    assert cg.params[0] == pytest.approx(13.100000015310233)
+    # add_cell! installs the file's cell as a new object (src/fileio.jl:58-61)
+    assert sg.cell.params[0] == pytest.approx(13.100000015310233)
     so, co, _ = diblock_system(O, (64, 1, 1), 1, "lamellar", (13.100000015310233,), chi=0.25)
     d, f = load_seed("lam1d")
     O.fieldinit(so, fields=f)
