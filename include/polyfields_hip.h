@@ -163,6 +163,18 @@ int pf_field_error(pf_system* sys, double* ferr);
 int pf_free_energy(pf_system* sys, double* fhelm);
 int pf_free_energy_bulk(pf_system* sys, double* fhelm);
 int pf_scfstress(pf_system* sys, double* stress, int nparams);
+/* pf_scfstress without device allocation.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc).  Every device buffer of the stress pass (the
+   row table, the per-row and k-space partial sums, the transforms of q_s and q+_s) is
+   carved from a fixed 32 MiB array in the library's code object on the system's GPU
+   (held until the process ends), so nothing is allocated or freed.  Requirements: the
+   system has run density! since its species last changed (the propagator tables exist),
+   its stacks are full (no checkpointing), and one stress row's transforms fit: grids up to
+   128^3 in FP32 and below 128^3 in FP64; beyond that it fails with -1 and a message giving
+   the bytes needed.  The array is shared by every call on that GPU, so calls that overlap
+   on one GPU race; the contents left by one call are never read by the next. */
+int pf_scfstress_noalloc(pf_system* sys, double* stress, int nparams);
 
 /* Free-energy functionals differentiated by pf_fhelm_grad. */
 #define PF_FHELM_REFERENCE 0 /* free_energy exactly as system.jl:402-433 (mu = phi) */

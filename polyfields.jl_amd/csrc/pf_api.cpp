@@ -384,6 +384,18 @@ int pf_scfstress(pf_system* sys, double* stress, int nparams) {
 }
 
 
+int pf_scfstress_noalloc(pf_system* sys, double* stress, int nparams) {
+    return guard([&] {
+        auto& s = S(sys);
+        need(stress && nparams == (int)s.cell.params.size(), "stress output");
+        PF_HIP(hipSetDevice(s.device));
+        size_t cap = 0;
+        char* scr = pf::noalloc_scratch(s.device, &cap);
+        auto v = s.scfstress_scratch(scr, cap);
+        for (int k = 0; k < nparams; ++k) stress[k] = v[k];
+    });
+}
+
 int pf_fhelm_grad(pf_system* sys, int functional, double* fhelm, double* dw, double* dcell,
                   int nparams, double* dchi, double* du, double* dphi, double* dN, int ndN) {
     return guard([&] {

@@ -281,6 +281,9 @@ struct ScftResults {
 ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o,
                  const std::function<void(int)>& save = {});
 
+// the allocation-free entry point's device scratch on `device` (pf_noalloc.hip)
+char* noalloc_scratch(int device, size_t* bytes);
+
 void* dev_alloc(size_t bytes);
 void dev_free(void* p);
 

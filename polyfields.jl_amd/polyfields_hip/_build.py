@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 OBJ = os.path.join(PKG_ROOT, "_build")
 LIB = os.path.join(HERE, "libpolyfields_hip.so")
 
-SOURCES = ["pf_fft.hip", "pf_fft_fixed.hip", "pf_kernels.hip", "pf_batch_kernels.hip", "pf_engine.hip", "pf_adjoint.hip", "pf_batch.cpp", "pf_system.cpp",
+SOURCES = ["pf_fft.hip", "pf_fft_fixed.hip", "pf_kernels.hip", "pf_batch_kernels.hip", "pf_engine.hip", "pf_adjoint.hip", "pf_noalloc.hip", "pf_batch.cpp", "pf_system.cpp",
            "pf_grad.cpp", "pf_tune.cpp", "pf_cell.cpp", "pf_julia_random.cpp", "pf_api.cpp"]
 
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")

@@ -70,6 +70,7 @@ SIGNATURES = {
     "pf_system_set_checkpoint": (I, [P, I]),
     "pf_free_energy_bulk": (I, [P, DP]),
     "pf_scfstress": (I, [P, DP, I]),
+    "pf_scfstress_noalloc": (I, [P, DP, I]),
     "pf_updater_anderson": (I, [PP, I, I, D]),
     "pf_updater_euler": (I, [PP, I, D]),
     "pf_updater_momentum": (I, [PP, I, D, D, D, D]),
