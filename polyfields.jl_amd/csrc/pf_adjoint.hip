@@ -451,22 +451,26 @@ template <typename T>
 bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                     hipStream_t st, int nextB) {
     if (schedule(sizeof(T), nx, ny, nz).slab == 0) return false;
-#define PF_AS(X, Y)                                                                         \
+#define PF_AS(X, Y, NT)                                                                     \
     if (nx == X && ny == Y) {                                                               \
         if (mode == 0)                                                                      \
-            hipLaunchKernelGGL((k_adj_slab_fx<T, 0, X, Y, 256>), dim3(nz, nl), dim3(256), 0, \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 0, X, Y, NT>), dim3(nz, nl), dim3(NT), 0,   \
                                st, d, nz, t, nextB);                                        \
         else if (mode == 1)                                                                 \
-            hipLaunchKernelGGL((k_adj_slab_fx<T, 1, X, Y, 256>), dim3(nz, nl), dim3(256), 0, \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 1, X, Y, NT>), dim3(nz, nl), dim3(NT), 0,   \
                                st, d, nz, t, nextB);                                        \
         else                                                                                \
-            hipLaunchKernelGGL((k_adj_slab_fx<T, 2, X, Y, 256>), dim3(nz, nl), dim3(256), 0, \
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 2, X, Y, NT>), dim3(nz, nl), dim3(NT), 0,   \
                                st, d, nz, t, 1);                                            \
         PF_HIP(hipGetLastError());                                                          \
         return true;                                                                        \
     }
-    PF_AS(32, 32)
-    PF_AS(64, 64)
+    PF_AS(32, 32, 256)
+    // FP64 64x64: 512 threads per plane (and per ky row in the z pass) make a whole 64^3
+    // gradient 15% faster on MI355X than 256; the LDS per workgroup is the same, the extra
+    // waves hide load latency, and the results are bitwise the same.
+    if constexpr (sizeof(T) == 8) { PF_AS(64, 64, 512) }
+    PF_AS(64, 64, 256)
 #undef PF_AS
     return false;
 }
@@ -474,15 +478,16 @@ bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, 
 template <typename T>
 int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hipStream_t st) {
     if (schedule(sizeof(T), nx, ny, nz).pencil == 0) return 0;
-#define PF_AP(X, Z, KB)                                                                     \
+#define PF_AP(X, Z, KB, NT)                                                                 \
     if (nx == X && nz == Z && ny % KB == 0) {                                               \
-        hipLaunchKernelGGL((k_adj_pencil_fx<T, Z, X / 2 + 1, KB, 256>), dim3(ny / KB, nl),    \
-                           dim3(256), 0, st, d, ny, t);                                     \
+        hipLaunchKernelGGL((k_adj_pencil_fx<T, Z, X / 2 + 1, KB, NT>), dim3(ny / KB, nl),     \
+                           dim3(NT), 0, st, d, ny, t);                                      \
         PF_HIP(hipGetLastError());                                                          \
         return ny / KB;                                                                     \
     }
-    PF_AP(32, 32, 2)
-    PF_AP(64, 64, 1)
+    PF_AP(32, 32, 2, 256)
+    if constexpr (sizeof(T) == 8) { PF_AP(64, 64, 1, 512) }
+    PF_AP(64, 64, 1, 256)
 #undef PF_AP
     return 0;
 }

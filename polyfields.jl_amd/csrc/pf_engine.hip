@@ -565,8 +565,7 @@ class Engine : public EngineBase {
             launch_point_density(s.w + (size_t)slot * n, s.mon_vol[slot], n, sp.rho_sp,
                                  s.partial, nparts, s.sc, (int)i, st);
         }
-        if (!tasks.empty())
-            launch_simpson<T>(d_tasks, (int)tasks.size(), n, st, n % (16 / sizeof(T)) == 0);
+        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), n, st);
     }
 
     // Stress scratch (src/species/multiblock.jl:233-295): the row table, the per-row sums,
@@ -845,7 +844,7 @@ class Engine : public EngineBase {
         for (auto& t : tk) t.rho_blk = dst[t.isp];
         auto* d = (SimpsonTask<T>*)dev_alloc(sizeof(SimpsonTask<T>) * tk.size());
         upload(d, tk.data(), sizeof(SimpsonTask<T>) * tk.size());
-        launch_simpson<T>(d, (int)tk.size(), s.n, s.stream, s.n % (16 / sizeof(T)) == 0);
+        launch_simpson<T>(d, (int)tk.size(), s.n, s.stream);
         PF_HIP(hipStreamSynchronize(s.stream));
         dev_free(d);
     }
@@ -1280,9 +1279,7 @@ class BatchEngine : public BatchEngineBase {
         const int nparts = reduce_parts(nmax);
         if (!tasks.empty()) {
             launch_species_Q<T>(d_tasks, (int)tasks.size(), nmax, partial, nparts, st);
-            bool vec = true;
-            for (const auto& tk : tasks) vec = vec && tk.n % (16 / sizeof(T)) == 0;
-            launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st, vec);
+            launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
         }
     }
 };

@@ -897,14 +897,18 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
         }
     }
     PF_SL1(64, 64, 512)
-    // 128 x 128 planes: 256 threads (two planes per CU), 512 (FP32), 1024 (FP32, one plane
-    // per CU with four times the threads; the FP32 default, measured on MI355X on the
-    // 128^3 blend: 13.4 -> 11.5 ms per density! with the 1024-thread pencil)
-    if constexpr (sizeof(T) == 4) {
-        if (sk == 2) { PF_SL1(128, 128, 512) }
-        if (sk == 4 || sk < 0) { PF_SL1(128, 128, 1024) }
+    // 128 x 128 planes: 256 threads (form 1), 512 (form 2, the FP64 default: one plane of
+    // 133 KiB per CU either way, with two waves per SIMD instead of one; measured on MI355X
+    // on the 128^3 blend in FP64, 23.9 -> 19.4 ms per density! with the 512-thread pencil),
+    // 1024 (form 4, FP32 only, whose default it is: the FP64 form spills; measured on the
+    // FP32 blend, 13.4 -> 11.5 ms per density! with the 1024-thread pencil)
+    if (nx == 128 && ny == 128) {
+        if (sk == 1) { PF_SL1(128, 128, 256) }
+        if constexpr (sizeof(T) == 4) {
+            if (sk == 4 || sk < 0) { PF_SL1(128, 128, 1024) }
+        }
+        PF_SL1(128, 128, 512)
     }
-    PF_SL(128, 128)
 #undef PF_SL
 #undef PF_SL1
     return false;
@@ -956,19 +960,22 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
     // the workgroups; measured on MI355X, one 64^3 system's density! 2.83 -> 2.42 ms)
     if (pk == 4 || (pk < 0 && sizeof(T) == 8)) { PF_PE(64, 64, 1) }
     PF_PE(64, 64, 2)
-    if constexpr (sizeof(T) == 4) {
-        // 128-point lines: the LDS pencil with 512 or 1024 threads
+    // 128-point lines: the LDS pencil with 512 threads (form 4, the FP64 default: 23.9 ->
+    // 22.0 ms per FP64 blend density! on its own) or 1024 (form 5, FP32 only, its default)
 #define PF_PE_NT(NT)                                                                        \
-        if (nx == 128 && nz == 128) {                                                       \
-            hipLaunchKernelGGL((k_pencil_fx<T, 128, 65, 1, NT>), dim3(ny, nl), dim3(NT), 0,    \
-                               st, d, ny, t);                                               \
-            PF_HIP(hipGetLastError());                                                      \
-            return true;                                                                    \
-        }
+    if (nx == 128 && nz == 128) {                                                           \
+        hipLaunchKernelGGL((k_pencil_fx<T, 128, 65, 1, NT>), dim3(ny, nl), dim3(NT), 0, st,    \
+                           d, ny, t);                                                       \
+        PF_HIP(hipGetLastError());                                                          \
+        return true;                                                                        \
+    }
+    if constexpr (sizeof(T) == 4) {
         if (pk == 4) { PF_PE_NT(512) }
         if (pk == 5 || pk < 0) { PF_PE_NT(1024) }
-#undef PF_PE_NT
+    } else {
+        if (pk == 4 || pk < 0) { PF_PE_NT(512) }
     }
+#undef PF_PE_NT
     PF_PE(128, 128, 1)
 #undef PF_PE
     return false;

@@ -180,7 +180,7 @@ struct Schedule {
                        // 3 shuffle-exchange (FP32 64 x 64 planes), 4 1024 threads (FP32 128^2)
     int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil,
                        // 4 LDS pencil with one ky row per workgroup (64-point lines) or
-                       // 512 threads (FP32 128-point lines), 5 with 1024 (FP32 128)
+                       // 512 threads (128-point lines), 5 with 1024 (FP32 128)
     int planar = -1;   // 0 generic, compile-time planar kernel: 1 (2-D: 256 threads),
                        // 2 (2-D: 512), 3 (2-D: 1024)
     int split = -1;    // streams per 3-D lane group in a batch (1 or 2)
@@ -272,10 +272,8 @@ void launch_fill(T* p, T v, size_t n, hipStream_t st);
 template <typename T>
 void launch_species_Q(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, double* partial,
                       int nparts, hipStream_t st);
-// vec: every task's n is a multiple of 16 / sizeof(T) (16-byte slice loads)
 template <typename T>
-void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st,
-                    bool vec = false);
+void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st);
 void launch_point_density(const double* w, double vol, size_t n, double* rho_sp,
                           double* partial, int nparts, DevScalars* sc, int isp,
                           hipStream_t st);

@@ -166,93 +166,10 @@ __global__ void k_simpson(const SimpsonTask<T>* __restrict__ tasks) {
     }
 }
 
-// The same sums with V = 16 / sizeof(T) consecutive points per thread and 16-byte loads of
-// every slice (grids whose n is a multiple of V): the per-point operations and their order
-// are those of k_simpson, so the densities are identical.  The slice loop is unrolled so
-// that several slices' loads are in flight per thread.
 template <typename T>
-struct Vec16;
-template <>
-struct Vec16<float> {
-    using type = float __attribute__((ext_vector_type(4)));
-    static constexpr int V = 4;
-};
-template <>
-struct Vec16<double> {
-    using type = double __attribute__((ext_vector_type(2)));
-    static constexpr int V = 2;
-};
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_simpson_v(const SimpsonTask<T>* __restrict__ tasks) {
-    using VT = typename Vec16<T>::type;
-    constexpr int V = Vec16<T>::V;
-    const SimpsonTask<T>* tk = tasks + blockIdx.y;
-    const int Ns = tk->Ns;
-    const size_t n = tk->n, nv = n / V;
-    const double QN = tk->sc->Q[tk->isp] * tk->Nref;
-    const bool rev = tk->qc == nullptr;
-    const auto q = (__attribute__((address_space(1))) const VT*)tk->q;
-    const auto qc = (__attribute__((address_space(1))) const VT*)(rev ? tk->q : tk->qc);
-    auto qv = [&](int s, size_t j) { return q[(size_t)(s - 1) * nv + j]; };
-    auto qcv = [&](int s, size_t j) {
-        return rev ? qc[(size_t)(Ns + 1 - s) * nv + j] : qc[(size_t)(s - 1) * nv + j];
-    };
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < nv;
-         j += (size_t)gridDim.x * blockDim.x) {
-        double acc_m[PF_MAXMON][V];
-        for (int m = 0; m < tk->nmon; ++m)
-#pragma unroll
-            for (int c = 0; c < V; ++c) acc_m[m][c] = 0.0;
-        for (int b = 0; b < tk->nblk; ++b) {
-            const int bgn = tk->begin[b], end = tk->begin[b + 1];
-            double rho[V];
-            {
-                const VT a0 = qv(bgn, j), c0 = qcv(bgn, j), a1 = qv(end, j), c1 = qcv(end, j);
-#pragma unroll
-                for (int c = 0; c < V; ++c) {
-                    rho[c] = (double)a0[c] * (double)c0[c];
-                    rho[c] += (double)a1[c] * (double)c1[c];
-                }
-            }
-#pragma unroll 4
-            for (int s = bgn + 1; s <= end - 1; s += 2) {
-                const VT a = qv(s, j), cc = qcv(s, j);
-#pragma unroll
-                for (int c = 0; c < V; ++c) rho[c] += 4.0 * (double)a[c] * (double)cc[c];
-            }
-#pragma unroll 4
-            for (int s = bgn + 2; s <= end - 2; s += 2) {
-                const VT a = qv(s, j), cc = qcv(s, j);
-#pragma unroll
-                for (int c = 0; c < V; ++c) rho[c] += 2.0 * (double)a[c] * (double)cc[c];
-            }
-            const int mon = tk->blk_mon[b];
-#pragma unroll
-            for (int c = 0; c < V; ++c) {
-                double r = rho[c] * (tk->ds[b] / 3.0);
-                r /= QN;
-                acc_m[mon][c] += r;
-                if (tk->rho_blk) tk->rho_blk[(size_t)b * n + j * V + c] = r;
-            }
-        }
-        for (int m = 0; m < tk->nmon; ++m)
-#pragma unroll
-            for (int c = 0; c < V; ++c) tk->rho_sp[(size_t)m * n + j * V + c] = acc_m[m][c];
-    }
-}
-
-template <typename T>
-void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st,
-                    bool vec) {
-    if (vec) {
-        constexpr int V = Vec16<T>::V;
-        int g = (int)std::min<size_t>((nmax / V + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_simpson_v<T>, dim3(g, ntasks), dim3(256), 0, st, d_tasks);
-    } else {
-        int g = (int)std::min<size_t>((nmax + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_simpson<T>, dim3(g, ntasks), dim3(256), 0, st, d_tasks);
-    }
+void launch_simpson(const SimpsonTask<T>* d_tasks, int ntasks, size_t nmax, hipStream_t st) {
+    int g = (int)std::min<size_t>((nmax + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_simpson<T>, dim3(g, ntasks), dim3(256), 0, st, d_tasks);
     PF_HIP(hipGetLastError());
 }
 
@@ -679,7 +596,7 @@ void launch_segsum(const double* in, int nseg, int nparts, int stride, int nvals
     template void launch_fill<T>(T*, T, size_t, hipStream_t);                                \
     template void launch_species_Q<T>(const SimpsonTask<T>*, int, size_t, double*, int,     \
                                       hipStream_t);                                         \
-    template void launch_simpson<T>(const SimpsonTask<T>*, int, size_t, hipStream_t, bool);
+    template void launch_simpson<T>(const SimpsonTask<T>*, int, size_t, hipStream_t);
 PF_INST2(float)
 PF_INST2(double)
 

@@ -97,11 +97,12 @@ def test_every_schedule_matches_default(precision, tol):
 
 
 def test_fp64_128_compile_time_matches_generic():
-    # the FP64 128 x 128 xy-slab and 128-line z-pencil (one workgroup per CU) against the
-    # generic kernels (schedule 0 / 0) on a 128^3 cubic diblock
+    # the FP64 128 x 128 xy-slab and 128-line z-pencil (one workgroup per CU) with 256
+    # threads (1 / 1), with 512 (2 / 4, the default) and the default schedule against the
+    # generic kernels (0 / 0) on a 128^3 cubic diblock
     dims = (128, 128, 128)
     out = []
-    for sched in ((0, 0), (-1, -1)):
+    for sched in ((0, 0), (1, 1), (2, 4), (-1, -1)):
         P.schedule_set("fp64", dims, slab=sched[0], pencil=sched[1])
         try:
             s, _, c = diblock_system(P, dims, 3, "cubic", (14.0,), N=100, f=0.35, chi=0.2, Ns=8,
@@ -112,10 +113,11 @@ def test_fp64_128_compile_time_matches_generic():
             del s
         finally:
             P.schedule_reset()
-    assert out[1][0] == pytest.approx(out[0][0], rel=1e-12)
-    for m in out[0][1]:
-        err = np.max(np.abs(out[1][1][m] - out[0][1][m])) / np.max(np.abs(out[0][1][m]))
-        assert err < 1e-12, (m, err)
+    for k in range(1, len(out)):
+        assert out[k][0] == pytest.approx(out[0][0], rel=1e-12), k
+        for m in out[0][1]:
+            err = np.max(np.abs(out[k][1][m] - out[0][1][m])) / np.max(np.abs(out[0][1][m]))
+            assert err < 1e-12, (k, m, err)
 
 
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("fp64", 1e-12)])

@@ -96,6 +96,49 @@ def test_blend_point_homopolymers_fp64():
     assert sg.free_energy_bulk() == pytest.approx(so.free_energy_bulk(), rel=1e-14)
 
 
+CONSTRAINTS = [("strict", 0.0), ("compressible", 0.5), ("unconstrained", -1.0)]
+CGRIDS = [((32, 1, 1), 1, "lamellar", (10.0,)), ((8, 8, 8), 3, "cubic", (4.0,))]
+
+
+@pytest.mark.parametrize("label,inv_zeta", CONSTRAINTS, ids=[c[0] for c in CONSTRAINTS])
+@pytest.mark.parametrize("dims,dim,crystal,params", CGRIDS, ids=["lam32", "cubic8"])
+def test_compressibility_and_edwards_residuals_fp64(label, inv_zeta, dims, dim, crystal,
+                                                    params):
+    # the three constraint modes of compress.jl:37-61 (eta from the density sum, with
+    # 1 / inv_zeta, or none) with Edwards excluded-volume potentials (edwards.jl:73-87) on
+    # both monomers beside the Flory pair: potentials, residuals, field error, free energy
+    # and a short Anderson run against the oracle
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        sys, cell, ch = diblock_system(mod, dims, dim, crystal, params, f=0.4, Ns=40,
+                                       compress=mod.Compressibility(inv_zeta), **extra)
+        ed = mod.EdwardsInteraction()
+        ed.set_interaction(1, 0.8)
+        ed.set_interaction(2, 0.3)
+        sys.add_interaction(ed)
+        return sys, cell, ch
+
+    shape = (dims[2], dims[1], dims[0])
+    (so, _, co), (sg, _, cg) = both(build, rand_fields(shape, scale=0.2, seed=9))
+    assert cg.Q == pytest.approx(co.Q, rel=1e-12)
+    for m in (1, 2):
+        assert rel(sg.density[m], so.density[m]) < 1e-12
+        assert rel(sg.potentials[m], so.potentials[m]) < 1e-12
+        assert rel(sg.residuals[m], so.residuals[m]) < 1e-11
+    assert sg.field_error() == pytest.approx(so.field_error(), rel=1e-11)
+    assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12, abs=1e-14)
+    assert sg.free_energy_bulk() == pytest.approx(so.free_energy_bulk(), rel=1e-14)
+    opts = dict(nsteps=15, ftol=1e-14)
+    upd = dict(nhist=4, iter_euler=3, lam_euler=0.05)
+    ro = O.scft(so, O.AndersonUpdater(**upd), opts=O.SCFTOptions(**opts))
+    rg = P.scft(sg, P.AndersonUpdater(**upd), opts=P.SCFTOptions(**opts))
+    assert rg.nsteps == ro.nsteps
+    assert rg.fhelm == pytest.approx(ro.fhelm, rel=1e-9)
+    assert rg.ferr == pytest.approx(ro.ferr, rel=1e-8)
+    for m in (1, 2):
+        assert rel(sg.fields[m], so.fields[m]) < 1e-8
+
+
 def test_anderson_trajectory_fp64():
     def build(mod, precision=None):
         extra = {} if precision is None else {"precision": precision}
@@ -223,7 +266,7 @@ def test_euler_momentum_updaters_fp64(kind):
         assert rel(sg.fields[m], so.fields[m]) < 1e-10
 
 
-@pytest.mark.parametrize("kind", ["MOM", "ADAM", "NAG"])
+@pytest.mark.parametrize("kind", ["MOM", "RMS", "ADAM", "NAG"])
 def test_scft_momentum_matches_oracle_fp64(kind):
     # scft! queues the next step's residuals! early for MOM / RMS / ADAM (their step starts
     # with residuals!), not for NAG; the loop must give the oracle's values either way
@@ -240,6 +283,27 @@ def test_scft_momentum_matches_oracle_fp64(kind):
     assert rg.ferr == pytest.approx(ro.ferr, rel=1e-8)
     for m in (1, 2):
         assert rel(sg.fields[m], so.fields[m]) < 1e-10
+
+
+@pytest.mark.parametrize("kind,lam", [("MOM", 0.05), ("ADAM", 0.005), ("NAG", 0.05)])
+def test_scft_momentum_converges_like_oracle_fp64(kind, lam):
+    # runs that reach ftol = 1e-3 before nsteps (176-222 steps on the oracle): the
+    # converged exit, with the queued residuals! of MOM / ADAM discarded, must stop at the
+    # oracle's step with its fields
+    def build(mod, precision=None):
+        extra = {} if precision is None else {"precision": precision}
+        return diblock_system(mod, (32, 1, 1), 1, "lamellar", (10.0,), **extra)
+
+    (so, _, _), (sg, _, _) = both(build, rand_fields((1, 1, 32), scale=0.1, seed=5))
+    opts = dict(nsteps=300, ftol=1e-3)
+    ro = O.scft(so, O.MomentumUpdater(method=kind, lam=lam), opts=O.SCFTOptions(**opts))
+    rg = P.scft(sg, P.MomentumUpdater(method=kind, lam=lam), opts=P.SCFTOptions(**opts))
+    assert ro.converged and ro.nsteps < 300
+    assert rg.converged and rg.nsteps == ro.nsteps
+    assert rg.fhelm == pytest.approx(ro.fhelm, rel=1e-9)
+    assert rg.ferr == pytest.approx(ro.ferr, rel=1e-6)
+    for m in (1, 2):
+        assert rel(sg.fields[m], so.fields[m]) < 1e-8
 
 
 def test_errors_are_raised_not_hidden():
