@@ -273,7 +273,9 @@ int pf_batch_set_graph(pf_batch* b, int on);
      3 shuffle-exchange (FP32 64 x 64 planes), 4 compile-time 1024 threads (FP32 128^2);
    z-pencil form: 0 generic, 1 compile-time LDS, 2 register-resident, 3 two threads per
      column (FP32, 64-point lines), 4 LDS with one ky row per workgroup (64-point lines) or
-     512 threads (128-point lines), 5 LDS with 1024 threads (FP32 128-point lines);
+     512 threads (128-point lines), 5 LDS with one ky row and 512 threads (64-point lines)
+     or 1024 threads (FP32 128-point lines), 6 LDS with half a ky row per workgroup (64-
+     and 128-point lines);
    in a batch, the streams per 3-D lane group (1 or 2);
    1-D/2-D planar kernel: 0 generic, 1 compile-time (2-D: 256 threads), 2 (2-D: 512),
      3 (2-D: 1024).

@@ -261,15 +261,20 @@ __global__ void __launch_bounds__(256) k_adj_pencil(const Lane<T>* __restrict__ 
 // Compile-time forms of the two reverse kernels for the production plane and line sizes:
 // the same arithmetic, order of operations and Simpson sources as k_adj_slab /
 // k_adj_pencil, with fixed radix plans, twiddles staged in LDS and unrolled loops.
-template <typename T, int MODE, int NX, int NY, int NT>
+//
+// SPLIT (modes 0 and 1): grid (2 nz, lanes).  Workgroups nz.. make the B spectrum of the
+// step whose A spectrum the first nz make (step t in mode 0, t + 1 in mode 1), as mode 2
+// does, so that a launch of few lanes has twice the workgroups; the same arithmetic.
+template <typename T, int MODE, int NX, int NY, int NT, bool SPLIT = false>
 __global__ void __launch_bounds__(NT) k_adj_slab_fx(const Lane<T>* __restrict__ lanes, int nz,
                                                     int t, int nextB) {
     constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
     __shared__ cpx<T> S[KPLANE];
     __shared__ cpx<T> TWX[NX];
     __shared__ cpx<T> TWY[NY];
-    // grid (nz, lanes): plane z of lane blockIdx.y
-    const int z = blockIdx.x;
+    // grid (nz, lanes): plane z of lane blockIdx.y (SPLIT: (2 nz, lanes), B part above nz)
+    const bool bpart = SPLIT && (int)blockIdx.x >= nz;
+    const int z = bpart ? (int)blockIdx.x - nz : (int)blockIdx.x;
     const Lane<T>* L = lanes + blockIdx.y;
     const int NS = L->nsteps;
     if (t >= NS) return;
@@ -304,6 +309,18 @@ __global__ void __launch_bounds__(NT) k_adj_slab_fx(const Lane<T>* __restrict__ 
         twiddles_in();
         to_spectrum(kB);
         return;
+    }
+    if constexpr (SPLIT) {
+        if (bpart) {
+            const int tb = MODE == 0 ? t : t + 1;
+            if (!nextB || tb >= NS) return;
+            b = step_block(L, NS - 1 - tb, &s_in);
+            lw_times_fwd(s_in, b);
+            twiddles_in();
+            to_spectrum(kB);
+            return;
+        }
+        nextB = 0;
     }
     if constexpr (MODE == 0) {
         const int s_out = s_in + L->dir;
@@ -447,6 +464,82 @@ __global__ void __launch_bounds__(NT) k_adj_pencil_fx(const Lane<T>* __restrict_
     }
 }
 
+// Reverse z pass over half of one ky row (columns [h CW, min((h + 1) CW, NXH)), CW =
+// ceil(NXH / 2)), for launches of few lanes: twice the workgroups of the one-row form,
+// the same arithmetic per column.  Partial-sum row r = blockIdx.x = 2 ky + h.  The second
+// half's missing column is loaded as a duplicate of a valid one, and neither summed nor
+// stored.
+template <typename T, int NZ, int NXH, int NT>
+__global__ void __launch_bounds__(NT) k_adj_pencil_cs(const Lane<T>* __restrict__ lanes, int ny,
+                                                      int t) {
+    constexpr int CW = (NXH + 1) / 2, CNT = NZ * CW, M = NXH - 1;
+    __shared__ cpx<T> PA[CNT];
+    __shared__ cpx<T> PB[CNT];
+    __shared__ cpx<T> TWZ[NZ];
+    const int r = blockIdx.x, ky = r >> 1, h = r & 1;
+    const int c0 = h * CW, cv = h ? NXH - CW : CW;
+    const Lane<T>* L = lanes + blockIdx.y;
+    const int NS = L->nsteps;
+    if (t >= NS) return;
+    int s_in;
+    const int b = step_block(L, NS - 1 - t, &s_in);
+    cpx<T>* kA = L->kbuf + (size_t)ky * NXH + c0;
+    const cpx<T>* kB = L->kb2 + (size_t)ky * NXH + c0;
+    const size_t zstride = (size_t)ny * NXH;
+#pragma unroll
+    for (int i0 = 0; i0 < CNT; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < CNT) {
+            const int c = i % CW;
+            const size_t g = (size_t)(i / CW) * zstride + (c < cv ? c : cv - 1);
+            PA[i] = kA[g];
+            PB[i] = kB[g];
+        }
+    }
+    lds_copy_in<T, NZ, NT>(TWZ, L->twz);
+    __syncthreads();
+    cfft<T, NZ, CW, CW, 1, NT, false, 1>(PA, TWZ);
+    cfft<T, NZ, CW, CW, 1, NT, false, 1>(PB, TWZ);
+    const double mc = -L->ldc[b];
+    const int npar = L->nparams;
+    const double jy = (double)fold(ky, ny);
+    double acc[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int i0 = 0; i0 < CNT; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < CNT && i % CW < cv) {
+            const int z = i / CW, kx = c0 + i % CW;
+            const double jz = (double)fold(z, NZ);
+            const double k2 = ksq_at(L->G, (double)kx, jy, jz);
+            const T f = exp_t<T>(mc * k2);
+            cpx<T>& A = PA[i];
+            const cpx<T> B = PB[i];
+            const double m = (kx == 0 || kx == M) ? 1.0 : 2.0;
+            const double g = m * ((double)A.x * (double)B.x + (double)A.y * (double)B.y) * (double)f;
+            for (int k = 0; k < npar; ++k)
+                acc[k] += g * dksq_at(L->dGG, npar, k, L->dim, (double)kx, jy, jz);
+            acc[PF_MAXPAR] += g * k2;
+            A.x *= f;
+            A.y *= f;
+        }
+    }
+    __syncthreads();
+    cfft<T, NZ, CW, CW, 1, NT, true, 1>(PA, TWZ);
+#pragma unroll
+    for (int i0 = 0; i0 < CNT; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < CNT && i % CW < cv) kA[(size_t)(i / CW) * zstride + i % CW] = PA[i];
+    }
+    double out[NG];
+    wg_sum<NG>(acc, out);
+    if (threadIdx.x == 0) {
+        double* gp = L->gpart + ((size_t)r * L->nblk + b) * NG;
+        for (int k = 0; k < NG; ++k) gp[k] += out[k];
+    }
+}
+
 template <typename T>
 bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                     hipStream_t st, int nextB) {
@@ -466,6 +559,19 @@ bool fixed_adj_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, 
         return true;                                                                        \
     }
     PF_AS(32, 32, 256)
+    // 64x64 planes in launches with at most one plane per CU (a few lanes): the A and B
+    // spectra from separate workgroups (SPLIT), twice the workgroups
+    if (nx == 64 && ny == 64 && mode != 2 && nl * nz <= device_cus()) {
+        constexpr int NT = sizeof(T) == 8 ? 512 : 256;
+        if (mode == 0)
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 0, 64, 64, NT, true>), dim3(2 * nz, nl),
+                               dim3(NT), 0, st, d, nz, t, nextB);
+        else
+            hipLaunchKernelGGL((k_adj_slab_fx<T, 1, 64, 64, NT, true>), dim3(2 * nz, nl),
+                               dim3(NT), 0, st, d, nz, t, nextB);
+        PF_HIP(hipGetLastError());
+        return true;
+    }
     // FP64 64x64: 512 threads per plane (and per ky row in the z pass) make a whole 64^3
     // gradient 15% faster on MI355X than 256; the LDS per workgroup is the same, the extra
     // waves hide load latency, and the results are bitwise the same.
@@ -486,6 +592,14 @@ int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hi
         return ny / KB;                                                                     \
     }
     PF_AP(32, 32, 2, 256)
+    // 64-point lines in launches with at most one ky row per CU: half rows (twice the
+    // workgroups and partial-sum rows; the sums then run in another order)
+    if (nx == 64 && nz == 64 && nl * ny <= device_cus()) {
+        hipLaunchKernelGGL((k_adj_pencil_cs<T, 64, 33, 256>), dim3(2 * ny, nl), dim3(256), 0, st,
+                           d, ny, t);
+        PF_HIP(hipGetLastError());
+        return 2 * ny;
+    }
     if constexpr (sizeof(T) == 8) { PF_AP(64, 64, 1, 512) }
     PF_AP(64, 64, 1, 256)
 #undef PF_AP

@@ -351,7 +351,7 @@ class Engine : public EngineBase {
             if (!fixed || !fixed_slab<T>(dl, 1, s.dims[0], ny, nz, SLAB_FWD, t0, st))
                 launch_slab<T>(dl, 1, nz, SLAB_FWD, t0, lds_plane, 256, st);
             for (int t = t0; t < t1; ++t) {
-                if (!fixed || !fixed_pencil<T>(dl, 1, s.dims[0], ny, nz, ntiles, tile, t, st))
+                if (!fixed || !fixed_pencil<T>(dl, 1, s.dims[0], ny, nz, true, t, st))
                     launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st);
                 if (!fixed || !fixed_slab<T>(dl, 1, s.dims[0], ny, nz, SLAB_INVFWD, t, st))
                     launch_slab<T>(dl, 1, nz, SLAB_INVFWD, t, lds_plane, 256, st);
@@ -545,7 +545,7 @@ class Engine : public EngineBase {
                 for (int t = 0; t < maxNs; ++t) {
                     const int nla = live_lanes(lanes, 0, nl, t);
                     timed(1, [&] {
-                        if (!fixed || !fixed_pencil<T>(d_lanes, nla, s.dims[0], ny, nz, ntiles, tile, t, st))
+                        if (!fixed || !fixed_pencil<T>(d_lanes, nla, s.dims[0], ny, nz, true, t, st))
                             launch_pencil<T>(d_lanes, nla, ny, ntiles, tile, t, lds_pencil, 256, st);
                     });
                     timed(0, [&] {
@@ -1263,7 +1263,7 @@ class BatchEngine : public BatchEngineBase {
                 for (int t = 0; t < gr.maxNs; ++t) {
                     const int nla = live_lanes(lanes, gr.off, gr.count, t);
                     timed(1, gr.st, nla, [&] {
-                        if (!fixed_pencil<T>(dl, nla, gr.nx, gr.ny, gr.nz, gr.ntiles, gr.tile, t, gr.st))
+                        if (!fixed_pencil<T>(dl, nla, gr.nx, gr.ny, gr.nz, false, t, gr.st))
                             launch_pencil<T>(dl, nla, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st);
                     });
                     timed(0, gr.st, nla, [&] {

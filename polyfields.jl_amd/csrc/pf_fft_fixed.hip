@@ -128,8 +128,9 @@ __device__ __forceinline__ void slab_forward_out(cpx<T>* S, cpx<T>* S2, const cp
         if (i0 + (int)threadIdx.x < KPLANE) kb[i0 + threadIdx.x] = S[i0 + threadIdx.x];
 }
 
-template <typename T, int NX, int NY, int MODE, int NT>
-__global__ void __launch_bounds__(NT) k_slab_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
+// MINW: the waves per SIMD the register budget must allow (launch bound), 1 = no limit
+template <typename T, int NX, int NY, int MODE, int NT, int MINW = 1>
+__global__ void __launch_bounds__(NT, MINW) k_slab_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
     constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
     // ping-pong transforms (one barrier per pass) where a second plane of LDS is cheap
     constexpr bool PPS = sizeof(T) == 4 && NX <= 64 && NY > 1 && cplan(NX / 2).npass % 2 == 0 &&
@@ -560,6 +561,73 @@ __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ la
     }
 }
 
+#define PF_CS64_NT 256
+#define PF_CS128_NT 512
+
+// z-pencil pass over half of one ky row: workgroup 2 ky + h takes the kx columns
+// [h CW, min((h + 1) CW, NXH)), CW = ceil(NXH / 2).  For launches of few lanes, where one
+// ky row per workgroup leaves CUs idle (one 64^3 system: 2 lanes x 64 rows), this gives
+// twice the workgroups at half the LDS each.  The second half has one column fewer: its
+// threads load a valid duplicate for it (no branch) and do not store it.
+template <typename T, int NZ, int NXH, int NT>
+__global__ void __launch_bounds__(NT) k_pencil_cs(const Lane<T>* __restrict__ lanes, int ny, int t) {
+    constexpr int CW = (NXH + 1) / 2;
+    __shared__ cpx<T> P[NZ * CW];
+    __shared__ cpx<T> TWZ[NZ];
+    // grid (2 ny, lanes)
+    const Lane<T>* L = lanes + blockIdx.y;
+    const int ky = blockIdx.x >> 1, h = blockIdx.x & 1;
+    const int c0 = h * CW, cv = h ? NXH - CW : CW;
+    unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf),
+                       tzr = __builtin_bit_cast(unsigned long long, L->twz);
+    int nsteps = L->nsteps, s_in;
+    int b = __builtin_amdgcn_readfirstlane(step_block(L, t, &s_in));
+    asm volatile("" : "+s"(kraw), "+s"(tzr), "+s"(nsteps), "+s"(b));
+    const auto kb = gp(__builtin_bit_cast(cpx<T>*, kraw) + (size_t)ky * NXH + c0);
+    const size_t zstride = (size_t)ny * NXH;
+    constexpr int CNT = NZ * CW, IT = (CNT + NT - 1) / NT, CH = 8;
+#pragma unroll
+    for (int q0 = 0; q0 < IT; q0 += CH) {
+        cpx<T> v[CH];
+        int o[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            if (q0 + q < IT) {
+                const int i0 = (q0 + q) * NT + (int)threadIdx.x, i = i0 < CNT ? i0 : i0 % CNT;
+                const int c = i % CW, cc = c < cv ? c : cv - 1;
+                o[q] = i;
+                v[q] = kb[(i / CW) * zstride + cc];
+            }
+#pragma unroll
+        for (int q = 0; q < CH; ++q)
+            if (q0 + q < IT) P[o[q]] = v[q];
+    }
+    lds_copy_in<T, NZ, NT>(TWZ, __builtin_bit_cast(const cpx<T>*, tzr));
+    if (t >= nsteps) return;
+    __syncthreads();
+    cfft<T, NZ, CW, CW, 1, NT, false, 1>(P, TWZ);
+    const double mc = -pick_blk(L->ldc, b);
+    const double* G = L->G;
+    const double fky = (double)fold(ky, ny);
+#pragma unroll
+    for (int i0 = 0; i0 < CNT; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < CNT) {
+            const int z = i / CW, kx = c0 + i % CW;
+            const T f = exp_t<T>(mc * ksq_at(G, (double)kx, fky, (double)fold(z, NZ)));
+            P[i].x *= f;
+            P[i].y *= f;
+        }
+    }
+    __syncthreads();
+    cfft<T, NZ, CW, CW, 1, NT, true, 1>(P, TWZ);
+#pragma unroll
+    for (int i0 = 0; i0 < CNT; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i < CNT && i % CW < cv) kb[(i / CW) * zstride + i % CW] = P[i];
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Register-resident transforms: one thread holds a whole line of N <= 64 values and
 // runs N = R1 * R2 as R2 transforms of length R1, twiddles w_N^(n2 k1), and R1 transforms
@@ -827,6 +895,19 @@ void schedule_reset() {
     sched_table.clear();
 }
 
+int device_cus() {
+    static std::mutex mu;
+    static std::map<int, int> cus;
+    int dev = 0;
+    PF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cus.find(dev);
+    if (it != cus.end()) return it->second;
+    int n = 0;
+    PF_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[dev] = n > 0 ? n : 1;
+}
+
 template <typename T>
 bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st) {
     const int pk = schedule(sizeof(T), nx, ny, 1).planar;
@@ -905,6 +986,17 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
     if (nx == 128 && ny == 128) {
         if (sk == 1) { PF_SL1(128, 128, 256) }
         if constexpr (sizeof(T) == 4) {
+            // FP32 default with more planes than CUs (three lanes of the 128^3 blend): 512
+            // threads within 128 VGPRs (a few spills), so that two planes (2 x 67 KiB of LDS)
+            // share a CU and the launch runs in one round instead of two; same arithmetic.
+            // Measured on MI355X: 128^3 FP32 blend 11.08 -> 10.78 ms per density!; the
+            // 1024-thread form within 64 VGPRs (two per CU, more spills) 12.56 ms.
+            if (sk < 0 && mode == SLAB_INVFWD && nl * nz > device_cus()) {
+                hipLaunchKernelGGL((k_slab_fx<T, 128, 128, SLAB_INVFWD, 512, 4>), dim3(nz, nl),
+                                   dim3(512), 0, st, d, nz, t);
+                PF_HIP(hipGetLastError());
+                return true;
+            }
             if (sk == 4 || sk < 0) { PF_SL1(128, 128, 1024) }
         }
         PF_SL1(128, 128, 512)
@@ -916,7 +1008,7 @@ bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int 
 
 // ky rows per workgroup: 4, 2, 1 for nx = 32, 64, 128 (NXH = 17, 33, 65)
 template <typename T>
-bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, int t,
+bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, bool alone, int t,
                   hipStream_t st) {
     const int pk = schedule(sizeof(T), nx, ny, nz).pencil;
     if (pk == 0) return false;
@@ -936,7 +1028,31 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
             return true;
         }
     }
-    if (pk != 1 && pk != 4 && pk != 5) {
+    // half ky rows (form 6): 64- and 128-point lines.  The default for a single system's
+    // launches of a few lanes, at most two ky rows per CU: FP64 128-point lines, FP32
+    // 128-point lines when one row per workgroup would take two rounds of the CUs, and FP32
+    // 64-point lines (otherwise the register pencil).  Measured on MI355X, 128^3 blend per
+    // launch: FP64 26.4 -> 22.6 us (2 lanes), 46.5 -> 31.6 us (3 lanes); FP32 26.4 -> 22.4
+    // us (3 lanes), but 15.6 -> 17.8 us for 2 lanes, which keep the 1024-thread row.  One
+    // 64^3 FP32 system's density! 1.64 -> 1.57 ms; no change for one 64^3 FP64 system.  In
+    // a batch, where another lane group's launches run beside these, the register pencil
+    // stays faster (FP32 64^3, 2 to 8 systems: 1% to 13%).  Each column's transform is the
+    // same arithmetic as in the one-row LDS forms.
+    const int rows = nl * ny, cus = device_cus();
+    const bool cs_default = pk < 0 && alone && rows <= 2 * cus &&
+                            ((nz == 128 && (sizeof(T) == 8 || rows > cus)) ||
+                             (nz == 64 && sizeof(T) == 4));
+    if (nx == nz && (nz == 64 || nz == 128) && (pk == 6 || cs_default)) {
+        if (nz == 64)
+            hipLaunchKernelGGL((k_pencil_cs<T, 64, 33, PF_CS64_NT>), dim3(2 * ny, nl),
+                               dim3(PF_CS64_NT), 0, st, d, ny, t);
+        else
+            hipLaunchKernelGGL((k_pencil_cs<T, 128, 65, PF_CS128_NT>), dim3(2 * ny, nl),
+                               dim3(PF_CS128_NT), 0, st, d, ny, t);
+        PF_HIP(hipGetLastError());
+        return true;
+    }
+    if (pk != 1 && pk != 4 && pk != 5 && pk != 6) {
         // one thread per column, the whole z line in registers
         const int ncol = (nx / 2 + 1) * ny;
 #define PF_PR(Z)                                                                            \
@@ -956,9 +1072,16 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int, int, in
 #undef PF_PR
     }
     PF_PE(32, 32, 4)
-    // 64-point lines: two ky rows per workgroup, or one (form 4, the FP64 default: twice
-    // the workgroups; measured on MI355X, one 64^3 system's density! 2.83 -> 2.42 ms)
-    if (pk == 4 || (pk < 0 && sizeof(T) == 8)) { PF_PE(64, 64, 1) }
+    // 64-point lines: two ky rows per workgroup, or one (form 4: twice the workgroups;
+    // measured on MI355X, one 64^3 FP64 system's density! 2.83 -> 2.42 ms), or one with 512
+    // threads (form 5, the FP64 default: 2.28 -> 2.15 ms, bitwise the same densities)
+    if (nx == 64 && nz == 64 && (pk == 5 || (pk < 0 && sizeof(T) == 8))) {
+        hipLaunchKernelGGL((k_pencil_fx<T, 64, 33, 1, 512>), dim3(ny, nl), dim3(512), 0, st, d,
+                           ny, t);
+        PF_HIP(hipGetLastError());
+        return true;
+    }
+    if (pk == 4) { PF_PE(64, 64, 1) }
     PF_PE(64, 64, 2)
     // 128-point lines: the LDS pencil with 512 threads (form 4, the FP64 default: 23.9 ->
     // 22.0 ms per FP64 blend density! on its own) or 1024 (form 5, FP32 only, its default)
@@ -985,8 +1108,8 @@ template bool fixed_planar<float>(const Lane<float>*, int, int, int, int, int, h
 template bool fixed_planar<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
 template bool fixed_slab<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t);
 template bool fixed_slab<double>(const Lane<double>*, int, int, int, int, int, int, hipStream_t);
-template bool fixed_pencil<float>(const Lane<float>*, int, int, int, int, int, int, int, hipStream_t);
-template bool fixed_pencil<double>(const Lane<double>*, int, int, int, int, int, int, int,
+template bool fixed_pencil<float>(const Lane<float>*, int, int, int, int, bool, int, hipStream_t);
+template bool fixed_pencil<double>(const Lane<double>*, int, int, int, int, bool, int,
                                    hipStream_t);
 
 }  // namespace pf

@@ -180,12 +180,15 @@ struct Schedule {
                        // 3 shuffle-exchange (FP32 64 x 64 planes), 4 1024 threads (FP32 128^2)
     int pencil = -1;   // 0 generic, 1 compile-time LDS pencil, 2 register pencil, 3 pair pencil,
                        // 4 LDS pencil with one ky row per workgroup (64-point lines) or
-                       // 512 threads (128-point lines), 5 with 1024 (FP32 128)
+                       // 512 threads (128-point lines), 5 one ky row and 512 threads
+                       // (64-point lines) or 1024 threads (FP32 128), 6 half a ky row
+                       // per workgroup (64- and 128-point lines)
     int planar = -1;   // 0 generic, compile-time planar kernel: 1 (2-D: 256 threads),
                        // 2 (2-D: 512), 3 (2-D: 1024)
     int split = -1;    // streams per 3-D lane group in a batch (1 or 2)
 };
 Schedule& schedule(int elem_bytes, int nx, int ny, int nz);
+int device_cus();        // compute units of the current device (looked up once per device)
 void schedule_reset();   // every grid back to the built-in defaults
 
 // compile-time specialised variants (pf_fft_fixed.hip); false: size not specialised
@@ -194,8 +197,9 @@ bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipS
 template <typename T>
 bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                 hipStream_t st);
+// alone: no other lane group's launches run beside this one (a single system's sweep)
 template <typename T>
-bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int ntiles, int tile, int t,
+bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, bool alone, int t,
                   hipStream_t st);
 
 // --- launchers (pf_kernels.hip) ---------------------------------------------------

@@ -35,14 +35,19 @@ static std::vector<Schedule> candidates(int elem, const int* dims, bool batch) {
     const int nz = dims[2];
     if (nz > 1) {
         // the pair pencil exists for FP32 z lines of 64, the shuffle-exchange slab for FP32
-        // 64 x 64 planes, the 512-thread LDS pencil for 128 x 128 x 128, and the 1024-thread
-        // slab and pencil for FP32 128 x 128 x 128
+        // 64 x 64 planes, the one-row LDS pencils (256 and 512 threads) for 64 x 64 x 64, the
+        // 512-thread LDS pencil for 128 x 128 x 128, and the 1024-thread slab and pencil for
+        // FP32 128 x 128 x 128
         const bool f32 = elem == 4;
         const bool p64 = f32 && dims[0] == 64 && dims[1] == 64;
         const bool p128 = f32 && dims[0] == 128 && dims[1] == 128;
         std::vector<int> pencils = {0, 1, 2};
         if (f32 && nz == 64) pencils.push_back(3);
-        if (nz == 64 && dims[0] == 64) pencils.push_back(4);
+        if (nz == 64 && dims[0] == 64) {
+            pencils.push_back(4);
+            pencils.push_back(5);
+        }
+        if ((nz == 64 || nz == 128) && dims[0] == nz) pencils.push_back(6);
         if (nz == 128 && dims[0] == 128) {
             pencils.push_back(4);
             if (f32) pencils.push_back(5);

@@ -40,7 +40,7 @@ def test_system_autotune(dims, dim, crystal, precision, tol):
     if dims[2] > 1:
         assert sched["slab"] in ((0, 1, 2, 3) if precision == "fp32" and dims[:2] == (64, 64)
                                  else (0, 1, 2))
-        assert sched["pencil"] in ((0, 1, 2, 3, 4) if dims[2] == 64 else (0, 1, 2))
+        assert sched["pencil"] in ((0, 1, 2, 3, 4, 5, 6) if dims[2] == 64 else (0, 1, 2))
     else:
         assert sched["planar"] in (0, 1, 2, 3)
     seeded(s)
@@ -76,7 +76,8 @@ def test_batch_autotune_then_scft():
 @pytest.mark.parametrize("precision,tol", [("fp32", 2e-5), ("fp64", 1e-12)])
 def test_every_schedule_matches_default(precision, tol):
     """Every slab x pencil candidate of a 64^3 grid (generic, compile-time and
-    shuffle-exchange slabs; generic, LDS, register and pair pencils) gives the default
+    shuffle-exchange slabs; generic, LDS (two rows, one row, one row with 512 threads, half
+    a row), register and pair pencils) gives the default
     schedule's densities to the transform rounding."""
     dims = (64, 64, 64)
     s, _, _ = diblock_system(P, dims, 3, "cubic", (8.0,), N=100, f=0.35, chi=0.2, Ns=12,
@@ -84,7 +85,7 @@ def test_every_schedule_matches_default(precision, tol):
     seeded(s, 9)
     s.density_update()
     ref = {m: s.density[m] for m in s.mids}
-    pencils = (0, 1, 2, 3, 4) if precision == "fp32" else (0, 1, 2, 4)
+    pencils = (0, 1, 2, 3, 4, 5, 6) if precision == "fp32" else (0, 1, 2, 4, 5, 6)
     slabs = (0, 1, 2, 3) if precision == "fp32" else (0, 1, 2)
     for slab in slabs:
         for pencil in pencils:
@@ -98,11 +99,11 @@ def test_every_schedule_matches_default(precision, tol):
 
 def test_fp64_128_compile_time_matches_generic():
     # the FP64 128 x 128 xy-slab and 128-line z-pencil (one workgroup per CU) with 256
-    # threads (1 / 1), with 512 (2 / 4, the default) and the default schedule against the
-    # generic kernels (0 / 0) on a 128^3 cubic diblock
+    # threads (1 / 1), with 512 (2 / 4, the default), the half-row pencil (2 / 6) and the
+    # default schedule against the generic kernels (0 / 0) on a 128^3 cubic diblock
     dims = (128, 128, 128)
     out = []
-    for sched in ((0, 0), (1, 1), (2, 4), (-1, -1)):
+    for sched in ((0, 0), (1, 1), (2, 4), (2, 6), (-1, -1)):
         P.schedule_set("fp64", dims, slab=sched[0], pencil=sched[1])
         try:
             s, _, c = diblock_system(P, dims, 3, "cubic", (14.0,), N=100, f=0.35, chi=0.2, Ns=8,
@@ -140,14 +141,15 @@ def test_every_planar_form_matches_default(precision, tol):
 
 
 def test_fp32_128_forms_match():
-    # the FP32 128^3 slab (256, 512, 1024 threads) and LDS pencil (256, 512, 1024 threads)
+    # the FP32 128^3 slab (256, 512, 1024 threads) and LDS pencil (256, 512, 1024 threads,
+    # half rows)
     dims = (128, 128, 128)
     s, _, _ = diblock_system(P, dims, 3, "cubic", (14.0,), N=100, f=0.35, chi=0.2, Ns=8,
                              precision="fp32")
     seeded(s, 6)
     s.density_update()
     ref = {m: s.density[m] for m in s.mids}
-    for slab, pencil in ((1, 1), (2, 4), (4, 5), (0, 0)):
+    for slab, pencil in ((1, 1), (2, 4), (4, 5), (4, 6), (0, 0)):
         P.schedule_set("fp32", dims, slab=slab, pencil=pencil)
         seeded(s, 6)
         s.density_update()
