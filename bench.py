@@ -199,18 +199,22 @@ def batched_scft(P, count, plain_sample):
     }
 
 
-# HBM bytes per lane and launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
-# 32-lane dispatch, FETCH x 2.0 and WRITE x 1.0 from a plain copy kernel of known size),
-# per xy-slab form: 3 the shuffle-exchange kernel (default), 2 the 512-thread LDS kernel.
-# Stored from the counter runs recorded in profiles/, not measured by this script.
-TRAFFIC_PER_LANE = {("xy-slab pass", 3): (2 * 31170.5 + 66327.6) * 1024 / 32,
-                    ("xy-slab pass", 2): (2 * 31193.8 + 66228.7) * 1024 / 32,
-                    ("z-pencil pass", None): (2 * 17091.8 + 33792.0) * 1024 / 32}
+# Memory bytes per lane and launch from rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B_sum and
+# TCC_EA0_WRREQ_WRITE_DRAM_32B_sum (read and write requests from the L2 to the memory
+# controller, in 32-byte units; median over the 32-lane dispatches), per xy-slab form: 3 the
+# shuffle-exchange kernel (default), 2 the 512-thread LDS kernel.  The same two counters
+# read exactly the bytes of a plain 68 MiB device copy (bench.py --workload copy).  Stored
+# from the counter runs recorded in profiles/, not measured by this script.
+TRAFFIC_PER_LANE = {("xy-slab pass", 3): (2137002 + 2132760) * 32 / 32,
+                    ("xy-slab pass", 2): (2138600 + 2129920) * 32 / 32,
+                    ("z-pencil pass", None): (1087608 + 1081344) * 32 / 32}
 
 
-TRAFFIC_SOURCE = ("stored: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this kernel "
-                  "form at this workload (profiles/propagator_64cubed_32inst_fp32.md), "
-                  "scaled by a copy kernel's reading; L2-to-memory bytes, not DRAM")
+TRAFFIC_SOURCE = ("stored: rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B_sum and "
+                  "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum runs of this kernel form at this workload "
+                  "(profiles/propagator_64cubed_32inst_fp32.md): bytes requested from the "
+                  "memory controller, which read a plain copy's bytes exactly; Infinity Cache "
+                  "hits are not told apart")
 
 
 def traffic_per_launch(kernel, args, slab_lanes, pen_lanes, slab_form):
@@ -440,6 +444,31 @@ def scft64_workload(P, args):
             "fp32_then_fp64_polish": pol}
 
 
+def copy_workload(P, args):
+    """Counter calibration: torch's device-to-device copy of a 68 MiB float32 buffer (the
+    size of one 32-lane xy-slab launch's working set), so that rocprofv3 --pmc readings of
+    this run give each counter's bytes per known byte read and written."""
+    import torch
+    nbytes = 68 * 2 ** 20
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / args.steps
+    return {"metric": "device copy bandwidth (read + write)", "value": 2 * nbytes / ms / 1e6,
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (ones)",
+            "config": {"workload": f"copy of {nbytes} bytes per step"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,10 +488,11 @@ def main():
                     help="density64: fix the 64^3 slab and pencil kernel forms (the numbers "
                          "of polyfields_hip.schedule_set) instead of autotuning")
     ap.add_argument("--workload", default="density64",
-                    choices=["density64", "blend128", "grad64", "scft64"],
+                    choices=["density64", "blend128", "grad64", "scft64", "copy"],
                     help="density64: the headline (64^3 batch); blend128 / grad64 / scft64: "
                          "the large blend, the free-energy gradient and the 64^3 "
-                         "variable-cell SCFT, one JSON line each")
+                         "variable-cell SCFT, one JSON line each; copy: a plain device "
+                         "copy of known size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT backend runs on a single MI355X "
@@ -473,7 +503,7 @@ def main():
 
     if args.workload != "density64":
         fn = {"blend128": blend_workload, "grad64": grad_workload,
-              "scft64": scft64_workload}[args.workload]
+              "scft64": scft64_workload, "copy": copy_workload}[args.workload]
         print(json.dumps(fn(P, args)))
         return
 
