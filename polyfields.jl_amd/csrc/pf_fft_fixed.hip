@@ -1029,19 +1029,21 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, bool alone, 
         }
     }
     // half ky rows (form 6): 64- and 128-point lines.  The default for a single system's
-    // launches of a few lanes, at most two ky rows per CU: FP64 128-point lines, FP32
-    // 128-point lines when one row per workgroup would take two rounds of the CUs, and FP32
-    // 64-point lines (otherwise the register pencil).  Measured on MI355X, 128^3 blend per
-    // launch: FP64 26.4 -> 22.6 us (2 lanes), 46.5 -> 31.6 us (3 lanes); FP32 26.4 -> 22.4
-    // us (3 lanes), but 15.6 -> 17.8 us for 2 lanes, which keep the 1024-thread row.  One
-    // 64^3 FP32 system's density! 1.64 -> 1.57 ms; no change for one 64^3 FP64 system.  In
-    // a batch, where another lane group's launches run beside these, the register pencil
-    // stays faster (FP32 64^3, 2 to 8 systems: 1% to 13%).  Each column's transform is the
-    // same arithmetic as in the one-row LDS forms.
+    // launches of a few lanes, at most two ky rows per CU, of FP64 128-point lines, and of
+    // FP32 128-point lines when one row per workgroup would take two rounds of the CUs.
+    // Measured on MI355X, 128^3 blend per launch: FP64 26.4 -> 22.6 us (2 lanes), 46.5 ->
+    // 31.6 us (3 lanes); FP32 26.4 -> 22.4 us (3 lanes), but 15.6 -> 17.8 us for 2 lanes,
+    // which keep the 1024-thread row.  One 64^3 FP64 system: no change.  One 64^3 FP32
+    // system: 1.64 -> 1.57 ms per density!, but not the default there: the variable-cell
+    // SCFT of that system in FP32 (64^3 BCC-biased seed, momentum + DomainUpdater) then
+    // ends above its stress tolerance of 3e-5 after 5000 steps, where the register pencil's
+    // rounding lets it converge (the Domain step's finite-difference Jacobian, dx = 1e-8,
+    // sits at the FP32 noise floor).  In a batch, where another lane group's launches run
+    // beside these, the register pencil stays faster (FP32 64^3, 2 to 8 systems: 1% to
+    // 13%).  Each column's transform is the same arithmetic as in the one-row LDS forms.
     const int rows = nl * ny, cus = device_cus();
-    const bool cs_default = pk < 0 && alone && rows <= 2 * cus &&
-                            ((nz == 128 && (sizeof(T) == 8 || rows > cus)) ||
-                             (nz == 64 && sizeof(T) == 4));
+    const bool cs_default = pk < 0 && alone && rows <= 2 * cus && nz == 128 &&
+                            (sizeof(T) == 8 || rows > cus);
     if (nx == nz && (nz == 64 || nz == 128) && (pk == 6 || cs_default)) {
         if (nz == 64)
             hipLaunchKernelGGL((k_pencil_cs<T, 64, 33, PF_CS64_NT>), dim3(2 * ny, nl),
