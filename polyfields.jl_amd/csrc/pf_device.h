@@ -67,6 +67,69 @@ __device__ __forceinline__ float exp_t<float>(double a) {
     return __builtin_amdgcn_exp2f((float)(a * 1.4426950408889634074));
 }
 
+// cos(2 pi m / 16) and cos(2 pi m / 12); sin(2 pi m / N) = cos(2 pi (m - N/4) / N)
+constexpr double kCos16[16] = {
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+constexpr double kCos12[12] = {
+    1.0, 0.86602540378443864676, 0.5, 0.0, -0.5, -0.86602540378443864676,
+    -1.0, -0.86602540378443864676, -0.5, 0.0, 0.5, 0.86602540378443864676};
+
+template <int N>
+constexpr double cos_frac(int m) {
+    static_assert(N == 16 || N == 12, "twiddle table");
+    m = ((m % N) + N) % N;
+    return N == 16 ? kCos16[m] : kCos12[m];
+}
+
+// v * w_N^m, w_N = exp(-/+ 2 pi i / N) (forward / inverse); m a constant after unrolling.
+// The quarter turns are exact sign swaps.
+template <typename T, int N, bool INV>
+__device__ __forceinline__ cpx<T> twmul(cpx<T> v, int m) {
+    if constexpr (N == 6) {
+        return twmul<T, 12, INV>(v, 2 * m);   // w_6^m = w_12^(2m)
+    } else {
+        m %= N;
+        if (m == 0) return v;
+        if (2 * m == N) return {-v.x, -v.y};
+        if (4 * m == N) return mul_i<T, INV>(v);
+        if (4 * m == 3 * N) return mul_i<T, !INV>(v);
+        const T c = T(cos_frac<N>(m)), sn = T(cos_frac<N>(m - N / 4));
+        const cpx<T> w = {c, INV ? sn : -sn};
+        return cmul(v, w);
+    }
+}
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void dft(cpx<T>* v);
+
+// Length N1 * N2 from N2 length-N1 and N1 length-N2 DFTs (Cooley-Tukey): input n = N2 n1 +
+// n2, output k = k1 + N1 k2, twiddles w_N^(n2 k1) between the stages.
+template <typename T, int N1, int N2, bool INV>
+__device__ __forceinline__ void dft_ct(cpx<T>* v) {
+    constexpr int N = N1 * N2;
+    cpx<T> y[N2][N1];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) y[n2][n1] = v[N2 * n1 + n2];
+        dft<T, N1, INV>(y[n2]);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) y[n2][k1] = twmul<T, N, INV>(y[n2][k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        cpx<T> z[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) z[n2] = y[n2][k1];
+        dft<T, N2, INV>(z);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[k1 + N1 * k2] = z[k2];
+    }
+}
+
 // Small DFTs of length R in registers.  Forward uses exp(-2 pi i/R), inverse exp(+...).
 template <typename T, int R, bool INV>
 __device__ __forceinline__ void dft(cpx<T>* v) {
@@ -112,6 +175,14 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
         v[6] = csub(e[2], o2);
         v[3] = cadd(e[3], o3);
         v[7] = csub(e[3], o3);
+    } else if constexpr (R == 6) {
+        dft_ct<T, 3, 2, INV>(v);
+    } else if constexpr (R == 12) {
+        dft_ct<T, 4, 3, INV>(v);
+    } else if constexpr (R == 16) {
+        dft_ct<T, 4, 4, INV>(v);
+    } else {
+        static_assert(R == 2, "radix");
     }
 }
 

@@ -14,8 +14,19 @@ struct CPlan {
     int ns[12];
 };
 
+// Radix plans: the greedy 8/4/2/3 factorisation, except for the lengths where a radix of
+// 6, 12 or 16 saves a pass (each pass is a round trip through LDS with two barriers):
+// 48 = 8 x 6, 96 = 8 x 12, 128 = 8 x 16, 256 = 16 x 16.
 constexpr CPlan cplan(int n) {
     CPlan p{};
+    if (n == 48 || n == 96 || n == 128 || n == 256) {
+        p.npass = 2;
+        p.r[0] = n == 256 ? 16 : 8;
+        p.ns[0] = 1;
+        p.r[1] = n / p.r[0];
+        p.ns[1] = p.r[0];
+        return p;
+    }
     int m = n, ns = 1;
     while (m % 8 == 0) { p.r[p.npass] = 8; p.ns[p.npass++] = ns; ns *= 8; m /= 8; }
     while (m % 4 == 0) { p.r[p.npass] = 4; p.ns[p.npass++] = ns; ns *= 4; m /= 4; }
@@ -33,7 +44,9 @@ template <typename T, int N, int R, int NS, int NL, int ES, int LS, int NT, bool
 __device__ __forceinline__ void cpass(const cpx<T>* src_buf, cpx<T>* dst_buf,
                                       const cpx<T>* __restrict__ tw) {
     constexpr int NB = N / R;
-    constexpr int BPR = (R == 8) ? 2 : (R == 2 ? 8 : 4);
+    // butterflies per thread and round: about 16 values in registers (32 in FP32 for the
+    // radices 12 and 16)
+    constexpr int BPR = R == 2 ? 8 : R == 8 || R == 6 ? 2 : R >= 12 ? (sizeof(T) == 4 ? 2 : 1) : 4;
     static_assert(NB <= BPR * NT, "transform too long for the workgroup");
     constexpr int LPR0 = (BPR * NT) / NB;
     constexpr int LPR = LPR0 < 1 ? 1 : (LPR0 > NL ? NL : LPR0);
