@@ -6,10 +6,14 @@ spline through the samples with zero second derivative at the end knots (the "Li
 boundary condition), i.e. the natural cubic spline; evaluated with scipy's
 CubicSpline(bc_type="natural") at range(1, n_in; length = n_out).
 
-Parity unpinned: the reference's only printed use (examples.ipynb, lam1d.json
-interpolated 64 -> 128, ferr 9.713161299981836e-6) is not reproduced by the current
-source's sampling range (which stretches the profile by 127/126 and gives ferr ~0.1); it
-is pinned here only by identity, exact reproduction of linear data and scipy's spline.
+Pinned by tests/test_oracle_interp_bspline.py: a restatement of Interpolations.jl
+0.13.1's published algorithm for that interpolant (B-spline prefilter with the Line
+boundary rows c_0 - 2c_1 + c_2 = 0, tensor-product evaluation, the reference's squeeze and
+sample points) agrees with this function to 1e-12 on 1-D, 2-D and 3-D grids; the Flat
+boundary rows (a planted mistake) do not.  The reference's only printed use (examples.ipynb,
+lam1d.json interpolated 64 -> 128, ferr 9.713161299981836e-6) is not reproduced by the
+current source's sampling range (which stretches the profile by 127/126 and gives ferr
+~0.1).
 """
 import json
 
