@@ -296,6 +296,20 @@ int pf_schedule_set(int elem_bytes, int nx, int ny, int nz, const int* sched);
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask /* nsys flags or NULL: all */,
                   pf_scft_results* results);
+/* pf_batch_scft with a variable cell: scft!(sys, AndersonUpdater(; nhist, iter_euler,
+   lam_euler); domain = DomainUpdater(...)) for every system in lockstep
+   (src/simulate.jl:60-148, src/updaters/domain.jl:50-120).  Each system gets its own
+   DomainUpdater with the settings of `domain` (nper, nskip, lam, pmax, rlxn and the
+   finite-difference step; `domain` itself is not set up or changed).  Every nskip steps
+   each active system's stress is taken from its last residuals! and, above stol, its
+   cell is relaxed by the Newton iterations, system by system; a system stops once
+   ferr < ftol and serr <= stol (src/simulate.jl:37-49).  results[i].serr is the stress
+   error the system's updater last computed, as scft! reports it.  A singular cell
+   Jacobian drops the system like a singular Anderson matrix.  No counterpart in the
+   reference beyond scft! itself. */
+int pf_batch_scft_domain(pf_batch* b, int nhist, int iter_euler, double lam_euler,
+                         const pf_domain* domain, int nsteps, double ftol, double stol,
+                         const int* mask, pf_scft_results* results);
 int pf_batch_timing(pf_batch* b, int on);
 int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches);
 /* Mean number of propagator lanes covered by the sampled launches of one kind (3-D

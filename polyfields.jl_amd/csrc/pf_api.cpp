@@ -663,20 +663,37 @@ int pf_batch_residuals(pf_batch* b) {
     });
 }
 
+static void copy_batch_results(const std::vector<pf::BatchResult>& r, pf_scft_results* out) {
+    for (size_t i = 0; i < r.size(); ++i) {
+        out[i].time = r[i].time;
+        out[i].nsteps = r[i].nsteps;
+        out[i].converged = r[i].converged;
+        out[i].ferr = r[i].ferr;
+        out[i].serr = r[i].serr;
+        out[i].fhelm = r[i].fhelm;
+    }
+}
+
 int pf_batch_scft(pf_batch* b, int nhist, int iter_euler, double lam_euler, int nsteps,
                   double ftol, const int* mask, pf_scft_results* results) {
     return guard([&] {
         need(results, "results");
         need(nhist >= 1, "nhist must be at least 1");
         auto r = B(b).scft_anderson(nhist, iter_euler, lam_euler, nsteps, ftol, mask);
-        for (size_t i = 0; i < r.size(); ++i) {
-            results[i].time = r[i].time;
-            results[i].nsteps = r[i].nsteps;
-            results[i].converged = r[i].converged;
-            results[i].ferr = r[i].ferr;
-            results[i].serr = 0.0;
-            results[i].fhelm = r[i].fhelm;
-        }
+        copy_batch_results(r, results);
+    });
+}
+
+int pf_batch_scft_domain(pf_batch* b, int nhist, int iter_euler, double lam_euler,
+                         const pf_domain* domain, int nsteps, double ftol, double stol,
+                         const int* mask, pf_scft_results* results) {
+    return guard([&] {
+        need(results, "results");
+        need(domain && domain->d, "domain");
+        need(nhist >= 1, "nhist must be at least 1");
+        auto r = B(b).scft_anderson(nhist, iter_euler, lam_euler, nsteps, ftol, mask,
+                                    domain->d, stol);
+        copy_batch_results(r, results);
     });
 }
 

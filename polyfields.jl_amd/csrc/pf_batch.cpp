@@ -286,14 +286,17 @@ static double fhelm_of(const System& s, const double* v) {
 }
 
 std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double lam_euler,
-                                              int nsteps, double ftol, const int* mask) {
+                                              int nsteps, double ftol, const int* mask,
+                                              const Domain* dom, double stol) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t ns = sys.size();
     std::vector<int> act;
     for (size_t i = 0; i < ns; ++i)
         if (!mask || mask[i]) act.push_back((int)i);
     std::vector<BatchResult> res(ns);
-    for (auto& r : res) r = {0, 0, std::nan(""), std::nan(""), 0.0, 0};
+    for (auto& r : res) r = {0, 0, std::nan(""), std::nan(""), 0.0, 0, 0.0};
+    const bool vcell = dom != nullptr;
+    if (vcell && dom->nskip < 1) throw Error(-1, "DomainUpdater nskip must be at least 1");
     if (act.empty()) return res;
     for (int i : act) {
         System* s = sys[i];
@@ -320,11 +323,27 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
     }
     std::vector<double> v;
     sys_sums(act, v);
+    // variable cell: every system gets its own DomainUpdater with the given settings,
+    // set up as scft! does after its first residuals! (src/simulate.jl:88-91); its stress
+    // reads the stacks of that residuals! pass
+    std::vector<double> serr(ns, 0.0);
+    dom_state.clear();
+    dom_state.resize(ns);
+    if (vcell) {
+        PF_HIP(hipStreamSynchronize(stream));
+        for (int i : act) {
+            dom_state[i].reset(new Domain(dom->nper, dom->nskip, dom->lam, dom->pmax, dom->rlxn));
+            dom_state[i]->fd_step = dom->fd_step;
+            dom_state[i]->setup(sys[i]);
+            serr[i] = dom_state[i]->stress_error();
+        }
+    }
     std::vector<int> next;
     for (int i : act) {
         const double fe = ferr_of(&v[(size_t)i * 5]);
-        res[i] = {fe < ftol, 0, fe, fhelm_of(*sys[i], &v[(size_t)i * 5]), 0.0, 0};
-        if (!(fe < ftol)) next.push_back(i);
+        const bool conv = scft_converged(fe, ftol, serr[i], stol, vcell);
+        res[i] = {conv, 0, fe, fhelm_of(*sys[i], &v[(size_t)i * 5]), 0.0, 0, serr[i]};
+        if (!conv) next.push_back(i);
     }
     act = next;
     int step = 0;
@@ -426,13 +445,36 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
         for (int i : act) fresh[i] = 0;
         if (iter >= nhist + 1)
             for (int i : act) and_state[i]->base = (and_state[i]->base + 1) % (nhist + 1);
+        // the cell update of scft! (src/simulate.jl:113-121), system by system through its
+        // own DomainUpdater: the stress of the stacks of this step's residuals!, and when
+        // serr > stol the Newton iterations, each ending with a residuals! at the new cell
+        if (vcell && step % dom->nskip == 0) {
+            PF_HIP(hipStreamSynchronize(stream));
+            for (int i : act) {
+                if (res[i].diverged) continue;
+                Domain& D = *dom_state[i];
+                try {
+                    D.scfstress_update();
+                    serr[i] = D.stress_error();
+                    if (serr[i] > stol) {
+                        D.step(D.nper, stol);
+                        serr[i] = D.stress_error();
+                    }
+                } catch (const Error&) {
+                    // scft! raises (a singular cell Jacobian, a negative Q ratio); in a
+                    // batch the system is reported as failed and dropped
+                    res[i].diverged = 1;
+                }
+                PF_HIP(hipStreamSynchronize(sys[i]->stream));
+            }
+        }
         sums_launch(act);
         // The next iteration's residuals (density! of the mixed fields) are queued before
         // the convergence test reads its sums, so the device does not wait for the host
         // between iterations.  Each system's residuals depend on its own fields only: for
         // the systems that stay active they are what the next iteration computes, and for
         // a system that the test drops they are the final residuals! of scft!.
-        if (step + 1 < nsteps) {
+        if (step + 1 < nsteps && !vcell) {
             residuals(act);
             queued = true;
             for (int i : act) fresh[i] = 1;
@@ -450,7 +492,7 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
                 // scft! raises "Diverging field updater" (src/simulate.jl:109-111); in a
                 // batch the system is reported (ferr = NaN, not converged) and dropped
                 res[i].diverged = 1;
-            } else if (fe < ftol) {
+            } else if (scft_converged(fe, ftol, serr[i], stol, vcell)) {
                 res[i].converged = 1;
             } else {
                 next.push_back(i);
@@ -477,6 +519,7 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
             res[i].ferr = ferr_of(&v[(size_t)i * 5]);
             res[i].fhelm = fhelm_of(*sys[i], &v[(size_t)i * 5]);
         }
+        res[i].serr = vcell ? dom_state[i]->stress_error() : 0.0;
         res[i].time = t;
     }
     PF_HIP(hipStreamSynchronize(stream));

@@ -91,6 +91,7 @@ struct BatchResult {
     int converged, nsteps;
     double ferr, fhelm, time;
     int diverged;
+    double serr;
 };
 
 // A few host threads that split a loop over [0, n) with the calling thread (the batch's
@@ -169,8 +170,12 @@ class Batch {
     void sums_launch(const std::vector<int>& act);
     void sums_read(const std::vector<int>& act, std::vector<double>& out);
     hipEvent_t sums_ev = nullptr;
+    // dom: the DomainUpdater settings for a variable cell (each system gets its own
+    // updater with these settings), or nullptr for a fixed cell
     std::vector<BatchResult> scft_anderson(int nhist, int iter_euler, double lam_euler,
-                                           int nsteps, double ftol, const int* mask);
+                                           int nsteps, double ftol, const int* mask,
+                                           const Domain* dom = nullptr, double stol = 0.0);
+    std::vector<std::unique_ptr<Domain>> dom_state;
 };
 
 std::unique_ptr<BatchEngineBase> make_batch_engine(Batch& b);

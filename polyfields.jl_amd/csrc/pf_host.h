@@ -280,6 +280,8 @@ struct ScftResults {
 // have, src/simulate.jl:140-142).  Empty: no saving.
 ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o,
                  const std::function<void(int)>& save = {});
+// isconverged (src/simulate.jl:37-49): ferr < ftol, and with a variable cell serr <= stol
+bool scft_converged(double ferr, double ftol, double serr, double stol, bool vcell);
 
 // the allocation-free entry point's device scratch on `device` (pf_noalloc.hip)
 char* noalloc_scratch(int device, size_t* bytes);

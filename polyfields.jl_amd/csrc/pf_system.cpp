@@ -804,7 +804,7 @@ void Domain::step(int nsteps, double /*tol: unused, as in the reference*/) {
 }
 
 // ---------------------------------------------------------------------------------
-static bool isconverged(double ferr, double ftol, double serr, double stol, bool vcell) {
+bool scft_converged(double ferr, double ftol, double serr, double stol, bool vcell) {
     // src/simulate.jl:37-49
     bool f = false, sc = true;
     if (ferr < ftol) {
@@ -830,7 +830,7 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o,
         d->setup(&s);
         serr = d->stress_error();
     }
-    bool conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
+    bool conv = scft_converged(ferr, o.ftol, serr, o.stol, vcell);
     // The next step's residuals! are queued before the host reads this step's errors
     // when nothing can change the fields or the cell in between: an updater whose step
     // starts with residuals! (Anderson, and Momentum other than NAG), a fixed cell, and the canonical ensemble (the grand free
@@ -864,7 +864,7 @@ ScftResults scft(System& s, Updater& u, Domain* d, const ScftOptions& o,
         }
         s.errors_read(&ferr, &fhelm);
         if (std::isnan(ferr)) throw Error(-5, "Diverging field updater -- breaking.");
-        conv = isconverged(ferr, o.ftol, serr, o.stol, vcell);
+        conv = scft_converged(ferr, o.ftol, serr, o.stol, vcell);
         if (conv) break;
         if (save_due) save(step);
         step += 1;

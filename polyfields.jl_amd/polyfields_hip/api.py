@@ -707,17 +707,25 @@ class Batch:
         for s in self.systems:
             s.precision = precision
 
-    def scft(self, nhist=10, iter_euler=1, lam_euler=0.01, nsteps=100, ftol=1e-5, mask=None):
-        """scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler)) for every system
-        (fixed cell), or for those with a true ``mask`` entry; returns one SCFTResults per
-        system (ferr NaN: diverged, or not run)."""
+    def scft(self, nhist=10, iter_euler=1, lam_euler=0.01, nsteps=100, ftol=1e-5, mask=None,
+             domain=None, stol=1e-5):
+        """scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler); domain) for every
+        system, or for those with a true ``mask`` entry; returns one SCFTResults per system
+        (ferr NaN: diverged, or not run).  ``domain``: a DomainUpdater whose settings each
+        system's own variable-cell updater takes (the object itself is not set up), or None
+        for a fixed cell; ``stol`` as in SCFTOptions."""
         res = (_CR * len(self.systems))()
         m = None
         if mask is not None:
             m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
             assert m.shape == (len(self.systems),)
-        call("pf_batch_scft", self._h, int(nhist), int(iter_euler), float(lam_euler),
-             int(nsteps), float(ftol), iptr(m) if m is not None else None, res)
+        if domain is not None:
+            call("pf_batch_scft_domain", self._h, int(nhist), int(iter_euler),
+                 float(lam_euler), domain._h, int(nsteps), float(ftol), float(stol),
+                 iptr(m) if m is not None else None, res)
+        else:
+            call("pf_batch_scft", self._h, int(nhist), int(iter_euler), float(lam_euler),
+                 int(nsteps), float(ftol), iptr(m) if m is not None else None, res)
         return [SCFTResults(r.time, r.nsteps, bool(r.converged), r.ferr, r.serr, r.fhelm)
                 for r in res]
 
