@@ -512,6 +512,72 @@ def scft64_workload(P, args):
             "fp32_then_fp64_polish": pol}
 
 
+# variable-cell sweep: the config-3 instances (FP64) with a relaxing hexagonal cell
+VC_DOMAIN = dict(nper=5, nskip=50, lam=0.1)
+VC_PH2 = dict(nhist=20, iter_euler=1, lam_euler=0.01, nsteps=1500, ftol=1e-6)
+VC_STOL = 1e-5
+
+
+def sweep_vcell_workload(P, args):
+    """A phase-diagram sweep with the cell relaxed: `--sweep-instances` of the batched
+    sweep's 2-D hexagonal instances (default 64 here), FP64, Euler mixing to ferr < 1e-2
+    on the fixed cell, then AndersonUpdater(nhist=20) with DomainUpdater(nper=5,
+    nskip=50) to ferr < 1e-6 and serr < 1e-5; batched (each DomainUpdater round of all
+    due systems in lockstep) against the same protocol one instance at a time on
+    `--plain-sample` of them."""
+    count = args.sweep_instances if args.sweep_instances != 256 else 64
+    cfg = sweep_instances(count)
+    systems = [sweep_system(P, c, "fp64") for c in cfg]
+    b = P.Batch(systems)
+    t0 = time.perf_counter()
+    r1 = b.scft(**PH1)
+    r2 = b.scft(mask=[int(r.converged) for r in r1], domain=P.DomainUpdater(**VC_DOMAIN),
+                stol=VC_STOL, **VC_PH2)
+    t_batch = time.perf_counter() - t0
+    solved = [i for i, r in enumerate(r2) if r.converged]
+    cells = [float(s.cell.params[0]) for s in systems]
+    moved = sum(1 for i in solved if abs(cells[i] - cfg[i][4]) > 1e-3 * cfg[i][4])
+    del b, systems
+    t_plain, n_plain, errs = 0.0, 0, []
+    for i, c in enumerate(cfg[:args.plain_sample]):
+        s = sweep_system(P, c, "fp64")
+        t0 = time.perf_counter()
+        ok = False
+        try:
+            q1 = P.scft(s, P.AndersonUpdater(nhist=10, iter_euler=10 ** 9, lam_euler=0.05),
+                        opts=P.SCFTOptions(nsteps=PH1["nsteps"], ftol=PH1["ftol"]))
+            if q1.converged:
+                q2 = P.scft(s, P.AndersonUpdater(nhist=20), domain=P.DomainUpdater(**VC_DOMAIN),
+                            opts=P.SCFTOptions(nsteps=VC_PH2["nsteps"], ftol=VC_PH2["ftol"],
+                                               stol=VC_STOL))
+                ok = q2.converged
+        except P.PolyFieldsError:
+            ok = False
+        t_plain += time.perf_counter() - t0
+        if ok:
+            n_plain += 1
+            if r2[i].converged:
+                errs.append({"fhelm_abs": abs(r2[i].fhelm - q2.fhelm),
+                             "a_rel": abs(cells[i] - float(s.cell.params[0])) / cells[i],
+                             "nsteps_batch": r2[i].nsteps, "nsteps_plain": q2.nsteps})
+    rate = len(solved) / t_batch
+    plain = n_plain / t_plain if t_plain > 0 else 0.0
+    return {"metric": "batched variable-cell SCFT solves/s (2-D hexagonal sweep, FP64)",
+            "value": rate, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded 0.1*N(0,1))",
+            "config": {"workload": f"{count} hexagonal diblock instances of the batched "
+                                   "sweep (grids 32^2-96^2), Euler lam=0.05 to ferr<1e-2 "
+                                   "(fixed cell), then Anderson nhist=20 + DomainUpdater("
+                                   "nper=5, nskip=50) to ferr<1e-6, serr<1e-5",
+                       "instances": count},
+            "solved": len(solved), "cells_moved": moved, "seconds": t_batch,
+            "plain_sample": args.plain_sample, "plain_solved": n_plain,
+            "plain_seconds": t_plain, "plain_solves_per_s": plain,
+            "speedup_vs_plain": rate / plain if plain > 0 else None,
+            "batch_vs_plain": errs}
+
+
 def copy_workload(P, args):
     """Counter calibration: torch's device-to-device copy of a 68 MiB float32 buffer (the
     size of one 32-lane xy-slab launch's working set), so that rocprofv3 --pmc readings of
@@ -556,7 +622,8 @@ def main():
                     help="density64: fix the 64^3 slab and pencil kernel forms (the numbers "
                          "of polyfields_hip.schedule_set) instead of autotuning")
     ap.add_argument("--workload", default="density64",
-                    choices=["density64", "blend128", "grad64", "scft64", "copy"],
+                    choices=["density64", "blend128", "grad64", "scft64", "sweep_vcell",
+                             "copy"],
                     help="density64: the headline (64^3 batch); blend128 / grad64 / scft64: "
                          "the large blend, the free-energy gradient and the 64^3 "
                          "variable-cell SCFT, one JSON line each; copy: a plain device "
@@ -571,7 +638,8 @@ def main():
 
     if args.workload != "density64":
         fn = {"blend128": blend_workload, "grad64": grad_workload,
-              "scft64": scft64_workload, "copy": copy_workload}[args.workload]
+              "scft64": scft64_workload, "sweep_vcell": sweep_vcell_workload,
+              "copy": copy_workload}[args.workload]
         print(json.dumps(fn(P, args)))
         return
 

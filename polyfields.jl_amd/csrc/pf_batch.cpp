@@ -285,6 +285,83 @@ static double fhelm_of(const System& s, const double* v) {
     return ftrans + (-v[2] / (double)s.n) + (v[3] / (double)s.n + v[4] / (double)s.n);
 }
 
+void Batch::stress_all(const std::vector<int>& set, std::vector<std::vector<double>>& out) {
+    PF_HIP(hipStreamSynchronize(stream));   // the batch launches wrote the stacks
+    for (int i : set) sys[i]->stress_launch();
+    for (int i : set) out[i] = sys[i]->stress_collect();
+}
+
+void Batch::domain_round(const std::vector<int>& due, double stol, std::vector<double>& serr,
+                         std::vector<BatchResult>& res) {
+    std::vector<std::vector<double>> st(sys.size());
+    std::vector<int> need;
+    {
+        std::vector<int> live;
+        for (int i : due)
+            if (!res[i].diverged) live.push_back(i);
+        stress_all(live, st);
+        for (int i : live) {
+            Domain& D = *dom_state[i];
+            D.stress = st[i];
+            serr[i] = D.stress_error();
+            if (serr[i] > stol) need.push_back(i);
+        }
+    }
+    if (need.empty()) return;
+    // step! (src/updaters/domain.jl:50-92) of every system in `need`: its first
+    // scfstress! reads the same stacks as the stress above, so that value is kept.  The
+    // cells change at every evaluation, so the launches run eagerly (no graph capture).
+    const bool graph_was = use_graph;
+    use_graph = false;
+    const int nper = dom_state[need[0]]->nper;
+    int maxnp = 0;
+    for (int i : need) maxnp = std::max(maxnp, (int)sys[i]->cell.params.size());
+    for (int it = 0; it < nper && !need.empty(); ++it) {
+        // dstress! (domain.jl:94-120): forward differences, one cell parameter at a time,
+        // the perturbed cells of all systems evaluated together
+        for (int i : need) dom_state[i]->oldparams = sys[i]->cell.params;
+        for (int k = 0; k < maxnp; ++k) {
+            std::vector<int> sub;
+            for (int i : need)
+                if ((int)sys[i]->cell.params.size() > k) sub.push_back(i);
+            for (int i : sub) {
+                Cell& c = sys[i]->cell;
+                c.params = dom_state[i]->oldparams;
+                c.params[k] += dom_state[i]->fd_step;
+                c.update();
+            }
+            density(sub);
+            stress_all(sub, st);
+            for (int i : sub) {
+                Domain& D = *dom_state[i];
+                const int np = (int)sys[i]->cell.params.size();
+                const double x = 1.0 / D.fd_step;
+                for (int r = 0; r < np; ++r) D.dstress[r * np + k] = x * (st[i][r] - D.stress[r]);
+            }
+        }
+        // the parameters are restored without update!, as the reference does
+        for (int i : need) sys[i]->cell.params = dom_state[i]->oldparams;
+        std::vector<int> ok;
+        for (int i : need) {
+            try {
+                dom_state[i]->newton_update();
+                ok.push_back(i);
+            } catch (const Error&) {
+                // step! raises on a singular Jacobian; in a batch the system is reported
+                // as failed and dropped
+                res[i].diverged = 1;
+            }
+        }
+        need = ok;
+        if (need.empty()) break;
+        residuals(need);
+        stress_all(need, st);
+        for (int i : need) dom_state[i]->stress = st[i];
+    }
+    use_graph = graph_was;
+    for (int i : need) serr[i] = dom_state[i]->stress_error();
+}
+
 std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double lam_euler,
                                               int nsteps, double ftol, const int* mask,
                                               const Domain* dom, double stol) {
@@ -330,12 +407,20 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
     dom_state.clear();
     dom_state.resize(ns);
     if (vcell) {
-        PF_HIP(hipStreamSynchronize(stream));
+        std::vector<std::vector<double>> st(ns);
+        stress_all(act, st);
         for (int i : act) {
-            dom_state[i].reset(new Domain(dom->nper, dom->nskip, dom->lam, dom->pmax, dom->rlxn));
-            dom_state[i]->fd_step = dom->fd_step;
-            dom_state[i]->setup(sys[i]);
-            serr[i] = dom_state[i]->stress_error();
+            Domain* D = new Domain(dom->nper, dom->nskip, dom->lam, dom->pmax, dom->rlxn);
+            dom_state[i].reset(D);
+            // setup! (src/updaters/domain.jl:28-37) with the stress computed above
+            const int np = (int)sys[i]->cell.params.size();
+            D->fd_step = dom->fd_step;
+            D->sys = sys[i];
+            D->stress = st[i];
+            D->oldparams.assign(np, 0.0);
+            D->stepv.assign(np, 0.0);
+            D->dstress.assign((size_t)np * np, 0.0);
+            serr[i] = D->stress_error();
         }
     }
     std::vector<int> next;
@@ -448,26 +533,7 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
         // the cell update of scft! (src/simulate.jl:113-121), system by system through its
         // own DomainUpdater: the stress of the stacks of this step's residuals!, and when
         // serr > stol the Newton iterations, each ending with a residuals! at the new cell
-        if (vcell && step % dom->nskip == 0) {
-            PF_HIP(hipStreamSynchronize(stream));
-            for (int i : act) {
-                if (res[i].diverged) continue;
-                Domain& D = *dom_state[i];
-                try {
-                    D.scfstress_update();
-                    serr[i] = D.stress_error();
-                    if (serr[i] > stol) {
-                        D.step(D.nper, stol);
-                        serr[i] = D.stress_error();
-                    }
-                } catch (const Error&) {
-                    // scft! raises (a singular cell Jacobian, a negative Q ratio); in a
-                    // batch the system is reported as failed and dropped
-                    res[i].diverged = 1;
-                }
-                PF_HIP(hipStreamSynchronize(sys[i]->stream));
-            }
-        }
+        if (vcell && step % dom->nskip == 0) domain_round(act, stol, serr, res);
         sums_launch(act);
         // The next iteration's residuals (density! of the mixed fields) are queued before
         // the convergence test reads its sums, so the device does not wait for the host

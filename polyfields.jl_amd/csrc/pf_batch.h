@@ -176,6 +176,15 @@ class Batch {
                                            int nsteps, double ftol, const int* mask,
                                            const Domain* dom = nullptr, double stol = 0.0);
     std::vector<std::unique_ptr<Domain>> dom_state;
+    // the stress of every system in `set` from its current stacks: the passes are queued
+    // on the systems' own streams, then collected (out[i] for i in set)
+    void stress_all(const std::vector<int>& set, std::vector<std::vector<double>>& out);
+    // one cell update of scft! (src/simulate.jl:113-121) for the systems in `due`, in
+    // lockstep: the stresses of their last residuals!, and for those above stol the
+    // DomainUpdater's Newton iterations, with every perturbed-cell density! and every
+    // residuals! of the systems evaluated together as batch launches
+    void domain_round(const std::vector<int>& due, double stol, std::vector<double>& serr,
+                      std::vector<BatchResult>& res);
 };
 
 std::unique_ptr<BatchEngineBase> make_batch_engine(Batch& b);

@@ -457,6 +457,7 @@ class Engine : public EngineBase {
 
     explicit Engine(System& s) : S(&s) {}
     ~Engine() override {
+        if (hrows) (void)hipHostFree(hrows);
         free_all();
         dev_free(tw);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -693,6 +694,13 @@ class Engine : public EngineBase {
     void stress_run(System& s, const StressBufs& bf, std::vector<double>& rows) {
         rows.assign((size_t)nrows * PF_MAXPAR, 0.0);
         if (!nrows) return;
+        stress_queue(s, bf, rows.data());
+        PF_HIP(hipStreamSynchronize(s.stream));
+    }
+
+    // the stress launches for buffers `bf` and the copy of the rows to host memory `dst`
+    // (nrows * PF_MAXPAR doubles), queued on the system's stream
+    void stress_queue(System& s, const StressBufs& bf, double* dst) {
         hipStream_t st = s.stream;
         if (ckK) {
             // checkpointed sweeps: each segment of q and q+ is recomputed from its
@@ -713,9 +721,30 @@ class Engine : public EngineBase {
         }
         if (!stress_fused())
             launch_segsum(bf.spart, nrows, s.dims[1] * ntiles, PF_MAXPAR, PF_MAXPAR, bf.srows, st);
-        PF_HIP(hipMemcpyAsync(rows.data(), bf.srows, sizeof(double) * nrows * PF_MAXPAR,
+        PF_HIP(hipMemcpyAsync(dst, bf.srows, sizeof(double) * nrows * PF_MAXPAR,
                               hipMemcpyDeviceToHost, st));
-        PF_HIP(hipStreamSynchronize(st));
+    }
+
+    double* hrows = nullptr;         // pinned copy of the stress rows (queued passes)
+    size_t hrows_len = 0;
+    void stress_rows_launch(System& s) override {
+        if (s.dirty_species || sb.size() != s.species.size()) build();
+        if (slanes_cell_version != s.cell.version || (int)slanes.size() != nrows) build_stress();
+        if (!nrows) return;
+        const size_t len = (size_t)nrows * PF_MAXPAR;
+        if (len > hrows_len) {
+            PF_HIP(hipStreamSynchronize(s.stream));
+            if (hrows) PF_HIP(hipHostFree(hrows));
+            hrows = nullptr;
+            PF_HIP(hipHostMalloc((void**)&hrows, sizeof(double) * len, hipHostMallocDefault));
+            hrows_len = len;
+        }
+        stress_queue(s, own_stress_bufs(), hrows);
+    }
+    void stress_rows_collect(System& s, std::vector<double>& rows) override {
+        PF_HIP(hipStreamSynchronize(s.stream));
+        rows.assign((size_t)nrows * PF_MAXPAR, 0.0);
+        if (nrows) std::copy(hrows, hrows + (size_t)nrows * PF_MAXPAR, rows.begin());
     }
 
     void stress_rows(System& s, std::vector<double>& rows) override {

@@ -131,6 +131,10 @@ struct System {
     std::vector<double> scfstress_scratch(char* scr, size_t cap);
     std::vector<double> stress_rows_combined(char* scr, size_t cap);
     std::vector<double> stress_combine(const std::vector<double>& rows) const;
+    // scfstress in two halves, so that many systems' stress passes run together: queue the
+    // pass on this system's stream, then wait for it and combine
+    void stress_launch();
+    std::vector<double> stress_collect();
     void monomer_fractions(std::vector<double>& fr);
     void sync();
     void set_precision(int p);       // swap the propagator engine (fields stay FP64)
@@ -179,6 +183,10 @@ class EngineBase {
     virtual ~EngineBase() = default;
     virtual void density(System& s) = 0;
     virtual void stress_rows(System& s, std::vector<double>& rows) = 0;   // [row][PF_MAXPAR]
+    // stress_rows in two halves: queue the pass and the copy of its rows to pinned host
+    // memory on the system's stream; wait for it and read the rows
+    virtual void stress_rows_launch(System& s) = 0;
+    virtual void stress_rows_collect(System& s, std::vector<double>& rows) = 0;
     // the same rows with every device buffer carved from the caller's `cap` bytes at scr
     virtual void stress_rows_scratch(System& s, std::vector<double>& rows, char* scr,
                                      size_t cap) = 0;
@@ -263,6 +271,9 @@ struct Domain {
     double stress_error() const;
     void step(int nsteps, double tol);
     void dstress_update();
+    // the Newton step on the cell from stress and dstress, its relaxation against
+    // oldparams, and update! of the cell (src/updaters/domain.jl:64-84)
+    void newton_update();
 };
 
 struct ScftOptions {

@@ -492,6 +492,25 @@ std::vector<double> System::stress_rows_combined(char* scr, size_t cap) {
     return stress_combine(rows);
 }
 
+void System::stress_launch() {
+    PF_HIP(hipSetDevice(device));
+    launch_muphi(sc, (int)species.size(), canonical, stream);
+    PF_HIP(hipMemcpyAsync(pinned, sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    eng->stress_rows_launch(*this);
+}
+
+std::vector<double> System::stress_collect() {
+    std::vector<double> rows;
+    eng->stress_rows_collect(*this, rows);   // waits for the stream, so for the scalars too
+    const DevScalars* h = reinterpret_cast<const DevScalars*>(pinned);
+    for (size_t i = 0; i < species.size(); ++i) {
+        Q[i] = h->Q[i];
+        phi[i] = h->phi[i];
+        mu[i] = h->mu[i];
+    }
+    return stress_combine(rows);
+}
+
 std::vector<double> System::scfstress(int exact) {
     if (exact) throw Error(-2, "only the reference stress formula is provided");
     return stress_rows_combined(nullptr, 0);
@@ -775,30 +794,33 @@ void Domain::dstress_update() {
     cell.params = oldparams;
 }
 
-void Domain::step(int nsteps, double /*tol: unused, as in the reference*/) {
-    System& s = *sys;
-    Cell& cell = s.cell;
+void Domain::newton_update() {
+    Cell& cell = sys->cell;
     const int np = (int)cell.params.size();
+    if (cell.dim == 1) {
+        for (int k = 0; k < np; ++k) stepv[k] = -1.0 * stress[0] / dstress[0];
+    } else {
+        std::vector<double> inv(np * np);
+        if (!lu_inverse(np, dstress.data(), inv.data()))
+            throw Error(-4, "singular cell Jacobian");
+        matvec(np, inv.data(), stress.data(), stepv.data());
+        for (auto& v : stepv) v *= -1.0;
+    }
+    const double size = norm2(stepv);
+    if (size > pmax)
+        for (auto& v : stepv) v *= pmax / size;
+    for (int k = 0; k < np; ++k) cell.params[k] += lam * stepv[k];
+    for (int k = 0; k < np; ++k)
+        cell.params[k] = rlxn * cell.params[k] + (1 - rlxn) * oldparams[k];
+    cell.update();
+}
+
+void Domain::step(int nsteps, double /*tol: unused, as in the reference*/) {
     scfstress_update();
     for (int it = 0; it < nsteps; ++it) {
         dstress_update();
-        if (cell.dim == 1) {
-            for (int k = 0; k < np; ++k) stepv[k] = -1.0 * stress[0] / dstress[0];
-        } else {
-            std::vector<double> inv(np * np);
-            if (!lu_inverse(np, dstress.data(), inv.data()))
-                throw Error(-4, "singular cell Jacobian");
-            matvec(np, inv.data(), stress.data(), stepv.data());
-            for (auto& v : stepv) v *= -1.0;
-        }
-        const double size = norm2(stepv);
-        if (size > pmax)
-            for (auto& v : stepv) v *= pmax / size;
-        for (int k = 0; k < np; ++k) cell.params[k] += lam * stepv[k];
-        for (int k = 0; k < np; ++k)
-            cell.params[k] = rlxn * cell.params[k] + (1 - rlxn) * oldparams[k];
-        cell.update();
-        s.residuals();
+        newton_update();
+        sys->residuals();
         scfstress_update();
     }
 }
