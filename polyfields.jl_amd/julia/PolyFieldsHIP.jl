@@ -244,6 +244,52 @@ function scft!(sys::HIPSystem, updater::HIPUpdater; domain = nothing,
 end
 
 # ---------------------------------------------------------------------------------------
+# Batches (no counterpart in the reference): many systems advanced together.
+
+mutable struct HIPBatch
+    handle::Ptr{Cvoid}
+    systems::Vector{HIPSystem}    # kept alive while the batch exists
+end
+
+function HIPBatch(systems::Vector{HIPSystem})
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    hs = Ptr{Cvoid}[s.handle for s in systems]
+    check(ccall((:pf_batch_create, LIB), Cint, (Ref{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Cint), h, hs,
+                length(hs)))
+    b = HIPBatch(h[], systems)
+    finalizer(x -> ccall((:pf_batch_destroy, LIB), Cint, (Ptr{Cvoid},), x.handle), b)
+    return b
+end
+
+"""
+    scft!(batch, nhist = 10; iter_euler = 1, lam_euler = 0.01, domain = nothing,
+          opts = SCFTOptions(), mask = nothing) -> Vector{SCFTResults}
+
+`scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler); domain, opts)` for every
+system of the batch in lockstep (`pf_batch_scft`, or `pf_batch_scft_domain` with a
+DomainUpdater, whose settings each system's own cell updater takes).  A diverged system
+is reported with ferr = NaN instead of raising.
+"""
+function scft!(b::HIPBatch, nhist::Integer = 10; iter_euler::Integer = 1,
+               lam_euler::Real = 0.01, domain = nothing, opts::SCFTOptions = SCFTOptions(),
+               mask = nothing)
+    r = Vector{PFResults}(undef, length(b.systems))
+    m = mask === nothing ? C_NULL : Cint.(mask)
+    if domain === nothing
+        check(ccall((:pf_batch_scft, LIB), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Float64, Cint, Float64, Ptr{Cint}, Ptr{PFResults}),
+                    b.handle, nhist, iter_euler, lam_euler, opts.nsteps, opts.ftol, m, r))
+    else
+        check(ccall((:pf_batch_scft_domain, LIB), Cint,
+                    (Ptr{Cvoid}, Cint, Cint, Float64, Ptr{Cvoid}, Cint, Float64, Float64,
+                     Ptr{Cint}, Ptr{PFResults}),
+                    b.handle, nhist, iter_euler, lam_euler, domain.handle, opts.nsteps,
+                    opts.ftol, opts.stol, m, r))
+    end
+    return [SCFTResults(x.time, x.nsteps, x.converged != 0, x.ferr, x.serr, x.fhelm) for x in r]
+end
+
+# ---------------------------------------------------------------------------------------
 # Reverse mode: d fhelm / d(w, cell, chi, u, phi, N_blk) (pf_fhelm_grad) and the
 # ChainRulesCore rrule that exposes it to Zygote / Diffractor.
 
