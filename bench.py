@@ -58,7 +58,7 @@ def seeded_fields(shape, seed):
     return {1: wa, 2: wb}
 
 
-def build(mod, precision=None, seed=0):
+def build(mod, precision=None, seed=0, chi=CHI):
     kw = {} if precision is None else {"precision": precision}
     cell = mod.UnitCell(3, "cubic", A_CELL)
     a, b = mod.Monomer(1), mod.Monomer(2)
@@ -66,7 +66,7 @@ def build(mod, precision=None, seed=0):
         warnings.simplefilter("ignore")
         ch = mod.Diblock(a, b, N, FA, 1.0, 1.0, 100)
     itx = mod.FloryInteraction()
-    itx.set_interaction(1, 2, CHI)
+    itx.set_interaction(1, 2, chi)
     sys_ = mod.FieldSystem((GRID, GRID, GRID), cell, monomers=[a, b], **kw)
     sys_.add_species(ch, 1.0)
     sys_.add_interaction(itx)
@@ -512,6 +512,66 @@ def scft64_workload(P, args):
             "fp32_then_fp64_polish": pol}
 
 
+def scft64_batch_workload(P, args):
+    """A sweep of the 64^3 variable-cell SCFT over chi N: `--instances` systems (default 8
+    here) with chi N = 18, 18.5, ... and seeds args.seed + k (BCC-biased fields), each
+    scft!(MomentumUpdater(MOM, lam=0.1); DomainUpdater(nskip=100, nper=10)) to
+    ftol = stol = 1e-5 in FP64, batched (one launch per contour step covers every
+    instance, the cell updates in lockstep) against the first `--plain-sample` of them one
+    at a time (default 1 here)."""
+    count = args.instances if args.instances != 32 else 8
+    sample = args.plain_sample if args.plain_sample != 8 else 1
+    chis = [(18.0 + 0.5 * k) / N for k in range(count)]
+
+    def make(k):
+        s, ch = build(P, "fp64", args.seed + k, chi=chis[k])
+        for m, w in seeded_fields(s.shape, args.seed + k).items():
+            s.fields[m] = w
+        return s
+
+    systems = [make(k) for k in range(count)]
+    b = P.Batch(systems)
+    t0 = time.perf_counter()
+    res = b.scft(updater=P.MomentumUpdater(method="MOM", lam=SCFT64["lam"]),
+                 domain=P.DomainUpdater(nskip=SCFT64["nskip"], nper=SCFT64["nper"]),
+                 nsteps=SCFT64["nsteps"], ftol=SCFT64["ftol"], stol=SCFT64["stol"])
+    t_batch = time.perf_counter() - t0
+    cells = [float(s.cell.params[0]) for s in systems]
+    del b, systems
+    t_plain, n_plain, cmp = 0.0, 0, []
+    for k in range(sample):
+        s = make(k)
+        t0 = time.perf_counter()
+        q = P.scft(s, P.MomentumUpdater(method="MOM", lam=SCFT64["lam"]),
+                   domain=P.DomainUpdater(nskip=SCFT64["nskip"], nper=SCFT64["nper"]),
+                   opts=P.SCFTOptions(nsteps=SCFT64["nsteps"], ftol=SCFT64["ftol"],
+                                      stol=SCFT64["stol"]))
+        t_plain += time.perf_counter() - t0
+        n_plain += int(q.converged)
+        cmp.append({"chiN": chis[k] * N, "nsteps_batch": res[k].nsteps, "nsteps_plain": q.nsteps,
+                    "fhelm_abs": abs(res[k].fhelm - q.fhelm),
+                    "a_rel": abs(cells[k] - float(s.cell.params[0])) / cells[k]})
+        del s
+    solved = sum(int(r.converged) for r in res)
+    rate = solved / t_batch
+    plain = n_plain / t_plain if t_plain > 0 else 0.0
+    return {"metric": "batched 64^3 variable-cell SCFT solves/s (chi N sweep, FP64)",
+            "value": rate, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+            "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded 0.1*N(0,1) + BCC cosine bias)",
+            "config": {"workload": f"{count} x 64^3 cubic AB diblock N=100 fA=0.35, chi N = "
+                                   f"18 .. {chis[-1] * N:g}, a0=4 Rg; scft! with "
+                                   "MomentumUpdater(MOM, lam=0.1) + DomainUpdater(nskip=100, "
+                                   "nper=10), ftol=stol=1e-5", "instances": count},
+            "solved": solved, "seconds": t_batch,
+            "per_instance": [{"chiN": chis[k] * N, "nsteps": r.nsteps, "converged": r.converged,
+                              "fhelm": r.fhelm, "a": cells[k]} for k, r in enumerate(res)],
+            "plain_sample": sample, "plain_solved": n_plain, "plain_seconds": t_plain,
+            "plain_solves_per_s": plain, "speedup_vs_plain": rate / plain if plain > 0 else None,
+            "batch_vs_plain": cmp}
+
+
 # variable-cell sweep: the config-3 instances (FP64) with a relaxing hexagonal cell
 VC_DOMAIN = dict(nper=5, nskip=50, lam=0.1)
 VC_PH2 = dict(nhist=20, iter_euler=1, lam_euler=0.01, nsteps=1500, ftol=1e-6)
@@ -622,8 +682,8 @@ def main():
                     help="density64: fix the 64^3 slab and pencil kernel forms (the numbers "
                          "of polyfields_hip.schedule_set) instead of autotuning")
     ap.add_argument("--workload", default="density64",
-                    choices=["density64", "blend128", "grad64", "scft64", "sweep_vcell",
-                             "copy"],
+                    choices=["density64", "blend128", "grad64", "scft64", "scft64_batch",
+                             "sweep_vcell", "copy"],
                     help="density64: the headline (64^3 batch); blend128 / grad64 / scft64: "
                          "the large blend, the free-energy gradient and the 64^3 "
                          "variable-cell SCFT, one JSON line each; copy: a plain device "
@@ -638,7 +698,8 @@ def main():
 
     if args.workload != "density64":
         fn = {"blend128": blend_workload, "grad64": grad_workload,
-              "scft64": scft64_workload, "sweep_vcell": sweep_vcell_workload,
+              "scft64": scft64_workload, "scft64_batch": scft64_batch_workload,
+              "sweep_vcell": sweep_vcell_workload,
               "copy": copy_workload}[args.workload]
         print(json.dumps(fn(P, args)))
         return

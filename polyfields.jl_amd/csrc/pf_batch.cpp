@@ -95,6 +95,7 @@ Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
     eng.reset();
+    free_momentum();
     dev_free(sums);
     dev_free(csums);
     dev_free(gpart);
@@ -362,9 +363,210 @@ void Batch::domain_round(const std::vector<int>& due, double stol, std::vector<d
     for (int i : need) serr[i] = dom_state[i]->stress_error();
 }
 
+namespace {
+
+// AndersonUpdater over the batch (src/updaters/anderson.jl:98-225): the Gram matrices on
+// the device, the least-squares solves on the host, the mix on the device
+struct AndersonStepper : BatchStepper {
+    int nhist, iter_euler;
+    double lam_euler;
+    bool deep = false;
+    AndersonStepper(int nh, int ie, double le) : nhist(nh), iter_euler(ie), lam_euler(le) {}
+    const char* noncanonical_error() const override {
+        return "Anderson mixing can only be used for systems in the canonical ensemble.";
+    }
+    void setup(Batch& b, const std::vector<int>& run) override {
+        b.alloc_step_table(nhist);
+        deep = nhist > PF_MAXHIST;
+        b.and_state.clear();
+        for (size_t i = 0; i < b.sys.size(); ++i) {
+            b.and_state.emplace_back(new Anderson(nhist, iter_euler, lam_euler));
+            if (std::find(run.begin(), run.end(), (int)i) == run.end()) continue;
+            Anderson& A = *b.and_state.back();
+            A.sys = b.sys[i];
+            const size_t N = b.sys[i]->n * b.sys[i]->nmon();
+            A.hist_r = (double*)dev_alloc(sizeof(double) * N * (nhist + 1));
+            A.hist_w = (double*)dev_alloc(sizeof(double) * N * (nhist + 1));
+            PF_HIP(hipMemcpyAsync(A.wslot(0), b.sys[i]->w, sizeof(double) * N,
+                                  hipMemcpyDeviceToDevice, b.stream));
+            A.is_setup = true;
+        }
+    }
+    void update(Batch& b, const std::vector<int>& act, std::vector<BatchResult>& res) override;
+};
+
+// MomentumUpdater (MOM, RMS, ADAM) over the batch (src/updaters/momentum.jl:47-139): one
+// kernel updates the moments and the fields of every active system
+struct MomentumStepper : BatchStepper {
+    int method;
+    double lam, b1, b2;
+    MomentumStepper(int m, double l, double bb1, double bb2) : method(m), lam(l), b1(bb1), b2(bb2) {}
+    const char* noncanonical_error() const override {
+        return "batched SCFT runs systems in the canonical ensemble only";
+    }
+    void setup(Batch& b, const std::vector<int>& run) override {
+        b.free_momentum();
+        b.mom_state.assign(b.sys.size(), Batch::MomState{});
+        for (int i : run) {
+            System& s = *b.sys[i];
+            const size_t N = s.n * s.nmon();
+            Batch::MomState& m = b.mom_state[i];
+            for (double** p : {&m.mt, &m.vt, &m.step})
+                *p = (double*)dev_alloc(sizeof(double) * N);
+            // setup! (momentum.jl:47-61): step = the first residual, zero moments
+            PF_HIP(hipMemcpyAsync(m.step, s.res, sizeof(double) * N, hipMemcpyDeviceToDevice,
+                                  b.stream));
+            launch_fill<double>(m.mt, 0.0, N, b.stream);
+            launch_fill<double>(m.vt, 0.0, N, b.stream);
+        }
+    }
+    void update(Batch& b, const std::vector<int>& act, std::vector<BatchResult>&) override {
+        size_t Nmax = 0;
+        for (int i : act) Nmax = std::max(Nmax, b.sys[i]->n * b.sys[i]->nmon());
+        if (act != b.mom_act || !b.d_mom) {
+            std::vector<MomTask> t;
+            for (int i : act) {
+                System& s = *b.sys[i];
+                const Batch::MomState& m = b.mom_state[i];
+                t.push_back({m.mt, m.vt, m.step, s.w, s.res, s.n * s.nmon()});
+            }
+            PF_HIP(hipStreamSynchronize(b.stream));   // queued work may read the old table
+            dev_free(b.d_mom);
+            b.d_mom = (MomTask*)dev_alloc(sizeof(MomTask) * t.size());
+            PF_HIP(hipMemcpyAsync(b.d_mom, t.data(), sizeof(MomTask) * t.size(),
+                                  hipMemcpyHostToDevice, b.stream));
+            PF_HIP(hipStreamSynchronize(b.stream));
+            b.mom_act = act;
+        }
+        launch_momentum_b(b.d_mom, (int)act.size(), method, lam, b1, b2, Nmax, b.stream);
+    }
+};
+
+void AndersonStepper::update(Batch& b, const std::vector<int>& act, std::vector<BatchResult>& res) {
+    // bookkeeping shared by the lockstep systems
+    Anderson& A0 = *b.and_state[act[0]];
+    const int iter = A0.iter + 1;
+    int ih;
+    double lam;
+    if (iter < nhist + 1) {
+        ih = iter - 1;
+        lam = 1.0 - std::pow(0.9, (double)iter);
+    } else {
+        ih = nhist;
+        lam = 1.0;
+    }
+    for (int i : act) {
+        b.and_state[i]->iter = iter;
+        b.and_state[i]->iter_hist = ih;
+        b.and_state[i]->lam = lam;
+    }
+    const bool euler = iter <= b.and_state[act[0]]->iter_euler;
+    for (size_t k = 0; k < act.size(); ++k) {
+        Anderson& A = *b.and_state[act[k]];
+        StepTask& s = b.h_step[k];
+        std::memset(&s, 0, sizeof(StepTask));
+        s.h = euler ? 0 : ih;
+        s.euler = euler;
+        s.lam = lam;
+        s.lam_euler = A.lam_euler;
+        s.r_out = A.rslot(ih);
+        if (s.h > 0) s.ring = A.ring(ih);
+        s.C = b.d_coef() + k * (size_t)b.coef_stride;
+        s.f_iter = A.wslot(ih);
+        s.f_out = (iter < nhist + 1) ? A.wslot(iter) : A.wslot(0);
+    }
+    // the StepTasks; the coefficients follow after the host solves below
+    PF_HIP(hipMemcpyAsync(b.d_step, b.h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice,
+                          b.stream));
+    if (!euler) {
+        const int nu = ih * (ih + 1) / 2;
+        const int nv = deep ? gram_tiles(ih) * 64 : nu + ih;
+        if (deep)
+            launch_gram_tiled_b(ih, b.d_act, b.d_step, (int)act.size(), b.gparts, b.gpart,
+                                b.csums, b.stream);
+        else
+            launch_gram_b(ih, b.d_act, b.d_step, (int)act.size(), max_parts(b.tasks, true),
+                          b.csums, b.stream);
+        PF_HIP(hipMemcpyAsync(b.hcsums, b.csums, sizeof(double) * nv * act.size(),
+                              hipMemcpyDeviceToHost, b.stream));
+        PF_HIP(hipStreamSynchronize(b.stream));
+        // each system's solve is independent: large batches split them over host threads
+        auto solve = [&](size_t k0, size_t k1) {
+            std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
+            for (size_t k = k0; k < k1; ++k) {
+                const double* g = b.hcsums + k * (size_t)nv;
+                double* hc = b.h_coef() + k * (size_t)b.coef_stride;
+                if (deep) {
+                    for (int m = 0; m < ih; ++m) {
+                        for (int nn = m; nn < ih; ++nn)
+                            U[m * ih + nn] = U[nn * ih + m] = g[gram_tile_index(m, nn)];
+                        V[m] = g[gram_tile_index(m, ih)];
+                    }
+                } else {
+                    int q = 0;
+                    for (int m = 0; m < ih; ++m)
+                        for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
+                    for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
+                }
+                if (!lu_inverse(ih, U.data(), inv.data())) {
+                    // scft! raises on a singular Gram matrix (anderson.jl:203-208); in a
+                    // batch the system is reported as failed and dropped
+                    res[act[k]].diverged = 1;
+                    for (int nn = 0; nn < ih; ++nn) hc[nn] = 0.0;
+                    continue;
+                }
+                matvec(ih, inv.data(), V.data(), C.data());
+                for (int nn = 0; nn < ih; ++nn) hc[nn] = C[nn];
+            }
+        };
+        if (b.pool && act.size() >= 64)
+            b.pool->run(act.size(), solve);
+        else
+            solve(0, act.size());
+        PF_HIP(hipMemcpyAsync(const_cast<double*>(b.d_coef()), b.h_coef(),
+                              sizeof(double) * b.coef_stride * act.size(), hipMemcpyHostToDevice,
+                              b.stream));
+    }
+    size_t Nmax = 0;
+    for (int i : act) Nmax = std::max(Nmax, b.sys[i]->n * b.sys[i]->nmon());
+    launch_mix_b(b.d_act, b.d_step, (int)act.size(), Nmax, b.stream);
+    if (iter >= nhist + 1)
+        for (int i : act) b.and_state[i]->base = (b.and_state[i]->base + 1) % (nhist + 1);
+}
+
+}  // namespace
+
+void Batch::free_momentum() {
+    for (auto& m : mom_state) {
+        dev_free(m.mt);
+        dev_free(m.vt);
+        dev_free(m.step);
+    }
+    mom_state.clear();
+    dev_free(d_mom);
+    d_mom = nullptr;
+    mom_act.clear();
+}
+
 std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double lam_euler,
                                               int nsteps, double ftol, const int* mask,
                                               const Domain* dom, double stol) {
+    AndersonStepper u(nhist, iter_euler, lam_euler);
+    return scft_run(u, nsteps, ftol, mask, dom, stol);
+}
+
+std::vector<BatchResult> Batch::scft_momentum(int method, double lam, double /*gamma: NAG only*/,
+                                              double b1, double b2, int nsteps, double ftol,
+                                              const int* mask, const Domain* dom, double stol) {
+    if (method < 0 || method > 2)
+        throw Error(-1, "batched MomentumUpdater: method must be MOM, RMS or ADAM (NAG "
+                        "evaluates its residuals at a look-ahead point)");
+    MomentumStepper u(method, lam, b1, b2);
+    return scft_run(u, nsteps, ftol, mask, dom, stol);
+}
+
+std::vector<BatchResult> Batch::scft_run(BatchStepper& u, int nsteps, double ftol,
+                                         const int* mask, const Domain* dom, double stol) {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t ns = sys.size();
     std::vector<int> act;
@@ -377,27 +579,14 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
     if (act.empty()) return res;
     for (int i : act) {
         System* s = sys[i];
-        if (!s->canonical) throw Error(-2, "Anderson mixing can only be used for systems in the canonical ensemble.");
+        if (!s->canonical) throw Error(-2, u.noncanonical_error());
         s->validate();
         s->sync();
     }
     const std::vector<int> run = act;
     PF_HIP(hipStreamSynchronize(stream));
-    alloc_step_table(nhist);
-    const bool deep = nhist > PF_MAXHIST;
     residuals(act);
-    and_state.clear();
-    for (size_t i = 0; i < ns; ++i) {
-        and_state.emplace_back(new Anderson(nhist, iter_euler, lam_euler));
-        if (std::find(run.begin(), run.end(), (int)i) == run.end()) continue;
-        Anderson& A = *and_state.back();
-        A.sys = sys[i];
-        const size_t N = sys[i]->n * sys[i]->nmon();
-        A.hist_r = (double*)dev_alloc(sizeof(double) * N * (nhist + 1));
-        A.hist_w = (double*)dev_alloc(sizeof(double) * N * (nhist + 1));
-        PF_HIP(hipMemcpyAsync(A.wslot(0), sys[i]->w, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
-        A.is_setup = true;
-    }
+    u.setup(*this, run);
     std::vector<double> v;
     sys_sums(act, v);
     // variable cell: every system gets its own DomainUpdater with the given settings,
@@ -433,109 +622,24 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
     act = next;
     int step = 0;
     bool queued = false;   // the residuals of the current active set are already queued
-    // fresh[i]: system i's last device work is a residuals pass queued after its last mix,
-    // which is then its final state (scft! ends with one residuals! after the last step)
+    // fresh[i]: system i's last device work is a residuals pass queued after its last
+    // update, which is then its final state (scft! ends with one residuals! after the
+    // last step)
     std::vector<char> fresh(ns, 0);
     while (!act.empty() && step < nsteps) {
-        // bookkeeping shared by the lockstep systems
-        Anderson& A0 = *and_state[act[0]];
-        const int iter = A0.iter + 1;
-        int ih;
-        double lam;
-        if (iter < nhist + 1) {
-            ih = iter - 1;
-            lam = 1.0 - std::pow(0.9, (double)iter);
-        } else {
-            ih = nhist;
-            lam = 1.0;
-        }
-        for (int i : act) {
-            and_state[i]->iter = iter;
-            and_state[i]->iter_hist = ih;
-            and_state[i]->lam = lam;
-        }
         if (queued)
             bind_active(act);
         else
             residuals(act);
         queued = false;
-        const bool euler = iter <= and_state[act[0]]->iter_euler;
-        for (size_t k = 0; k < act.size(); ++k) {
-            Anderson& A = *and_state[act[k]];
-            StepTask& s = h_step[k];
-            std::memset(&s, 0, sizeof(StepTask));
-            s.h = euler ? 0 : ih;
-            s.euler = euler;
-            s.lam = lam;
-            s.lam_euler = A.lam_euler;
-            s.r_out = A.rslot(ih);
-            if (s.h > 0) s.ring = A.ring(ih);
-            s.C = d_coef() + k * (size_t)coef_stride;
-            s.f_iter = A.wslot(ih);
-            s.f_out = (iter < nhist + 1) ? A.wslot(iter) : A.wslot(0);
-        }
-        // the StepTasks; the coefficients follow after the host solves below
-        PF_HIP(hipMemcpyAsync(d_step, h_step, sizeof(StepTask) * act.size(), hipMemcpyHostToDevice, stream));
-        if (!euler) {
-            const int nu = ih * (ih + 1) / 2;
-            const int nv = deep ? gram_tiles(ih) * 64 : nu + ih;
-            if (deep)
-                launch_gram_tiled_b(ih, d_act, d_step, (int)act.size(), gparts, gpart, csums,
-                                    stream);
-            else
-                launch_gram_b(ih, d_act, d_step, (int)act.size(), max_parts(tasks, true), csums,
-                              stream);
-            PF_HIP(hipMemcpyAsync(hcsums, csums, sizeof(double) * nv * act.size(), hipMemcpyDeviceToHost, stream));
-            PF_HIP(hipStreamSynchronize(stream));
-            // each system's solve is independent: large batches split them over host threads
-            auto solve = [&](size_t k0, size_t k1) {
-                std::vector<double> U(ih * ih), V(ih), inv(ih * ih), C(ih);
-                for (size_t k = k0; k < k1; ++k) {
-                    const double* g = hcsums + k * (size_t)nv;
-                    double* hc = h_coef() + k * (size_t)coef_stride;
-                    if (deep) {
-                        for (int m = 0; m < ih; ++m) {
-                            for (int nn = m; nn < ih; ++nn)
-                                U[m * ih + nn] = U[nn * ih + m] = g[gram_tile_index(m, nn)];
-                            V[m] = g[gram_tile_index(m, ih)];
-                        }
-                    } else {
-                        int q = 0;
-                        for (int m = 0; m < ih; ++m)
-                            for (int nn = m; nn < ih; ++nn) U[m * ih + nn] = U[nn * ih + m] = g[q++];
-                        for (int m = 0; m < ih; ++m) V[m] = g[nu + m];
-                    }
-                    if (!lu_inverse(ih, U.data(), inv.data())) {
-                        // scft! raises on a singular Gram matrix (anderson.jl:203-208); in a
-                        // batch the system is reported as failed and dropped
-                        res[act[k]].diverged = 1;
-                        for (int nn = 0; nn < ih; ++nn) hc[nn] = 0.0;
-                        continue;
-                    }
-                    matvec(ih, inv.data(), V.data(), C.data());
-                    for (int nn = 0; nn < ih; ++nn) hc[nn] = C[nn];
-                }
-            };
-            if (pool && act.size() >= 64)
-                pool->run(act.size(), solve);
-            else
-                solve(0, act.size());
-            PF_HIP(hipMemcpyAsync(const_cast<double*>(d_coef()), h_coef(),
-                                  sizeof(double) * coef_stride * act.size(),
-                                  hipMemcpyHostToDevice, stream));
-        }
-        size_t Nmax = 0;
-        for (int i : act) Nmax = std::max(Nmax, sys[i]->n * sys[i]->nmon());
-        launch_mix_b(d_act, d_step, (int)act.size(), Nmax, stream);
+        u.update(*this, act, res);
         for (int i : act) fresh[i] = 0;
-        if (iter >= nhist + 1)
-            for (int i : act) and_state[i]->base = (and_state[i]->base + 1) % (nhist + 1);
-        // the cell update of scft! (src/simulate.jl:113-121), system by system through its
-        // own DomainUpdater: the stress of the stacks of this step's residuals!, and when
-        // serr > stol the Newton iterations, each ending with a residuals! at the new cell
+        // the cell update of scft! (src/simulate.jl:113-121): the stress of the stacks of
+        // this step's residuals!, and when serr > stol the Newton iterations, each ending
+        // with a residuals! at the new cell (all due systems in lockstep)
         if (vcell && step % dom->nskip == 0) domain_round(act, stol, serr, res);
         sums_launch(act);
-        // The next iteration's residuals (density! of the mixed fields) are queued before
+        // The next iteration's residuals (density! of the updated fields) are queued before
         // the convergence test reads its sums, so the device does not wait for the host
         // between iterations.  Each system's residuals depend on its own fields only: for
         // the systems that stay active they are what the next iteration computes, and for

@@ -51,6 +51,15 @@ struct StepTask {
     int h, euler;
 };
 
+// Momentum family state of one system (device arrays of N = n * nmon doubles)
+struct MomTask {
+    double *mt, *vt, *step, *w;
+    const double* r;
+    size_t N;
+};
+void launch_momentum_b(const MomTask* d, int nsys, int method, double lam, double b1, double b2,
+                       size_t Nmax, hipStream_t st);
+
 template <typename T>
 void launch_lw_fill_b(const LwTask<T>* lw, int nlw, const FillTask<T>* fill, int nfill,
                       size_t nmax, hipStream_t st);
@@ -117,6 +126,19 @@ class HostPool {
     bool stop = false;
 };
 
+class Batch;
+// A field updater run over a batch by Batch::scft_run: every step starts with residuals!
+// of the active systems (done by the loop, possibly queued early), then update() moves
+// the fields of every active system.
+struct BatchStepper {
+    virtual ~BatchStepper() = default;
+    virtual const char* noncanonical_error() const = 0;
+    // setup! after the first residuals! of the systems in `run` (src/simulate.jl:86)
+    virtual void setup(Batch& b, const std::vector<int>& run) = 0;
+    // the rest of step! for the systems in `act`; may mark systems diverged in res
+    virtual void update(Batch& b, const std::vector<int>& act, std::vector<BatchResult>& res) = 0;
+};
+
 class Batch {
    public:
     std::vector<System*> sys;
@@ -175,6 +197,22 @@ class Batch {
     std::vector<BatchResult> scft_anderson(int nhist, int iter_euler, double lam_euler,
                                            int nsteps, double ftol, const int* mask,
                                            const Domain* dom = nullptr, double stol = 0.0);
+    // scft!(sys, MomentumUpdater(method, lam, gamma, beta1, beta2)) for method 0 MOM,
+    // 1 RMS, 2 ADAM (NAG evaluates its residuals at a look-ahead point, not batched)
+    std::vector<BatchResult> scft_momentum(int method, double lam, double gamma, double b1,
+                                           double b2, int nsteps, double ftol, const int* mask,
+                                           const Domain* dom = nullptr, double stol = 0.0);
+    // the scft! loop of src/simulate.jl:60-148 over the batch with updater u
+    std::vector<BatchResult> scft_run(BatchStepper& u, int nsteps, double ftol, const int* mask,
+                                      const Domain* dom, double stol);
+    // momentum state, per system (empty unless a momentum run set it up)
+    struct MomState {
+        double *mt = nullptr, *vt = nullptr, *step = nullptr;
+    };
+    std::vector<MomState> mom_state;
+    MomTask* d_mom = nullptr;
+    std::vector<int> mom_act;        // active set d_mom was built for
+    void free_momentum();
     std::vector<std::unique_ptr<Domain>> dom_state;
     // the stress of every system in `set` from its current stacks: the passes are queued
     // on the systems' own streams, then collected (out[i] for i in set)

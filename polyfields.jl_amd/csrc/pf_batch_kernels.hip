@@ -1,6 +1,7 @@
 // Batched system kernels: one launch serves every system of a batch (blockIdx.y is the
 // system or task).  Per-point arithmetic is the same as in pf_kernels.hip.
 #include "pf_batch.h"
+#include "pf_device.h"
 
 namespace pf {
 
@@ -349,6 +350,26 @@ __global__ void k_mix_b(const SysTask* __restrict__ tasks, const StepTask* __res
         w[i] = o;
         if (s->f_out) s->f_out[i] = o;
     }
+}
+
+// Momentum family step of every system (blockIdx.y): the update of src/updaters/
+// momentum.jl:86-139 from this step's residual, then w += lam * step, per point in the
+// one-system order (momentum update, then the axpy)
+__global__ void k_momentum_b(const MomTask* __restrict__ t, int method, double lam, double b1,
+                             double b2) {
+    const MomTask m = t[blockIdx.y];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m.N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        momentum_point(method, m.mt, m.vt, m.step, m.r, b1, b2, i);
+        m.w[i] += lam * m.step[i];
+    }
+}
+
+void launch_momentum_b(const MomTask* d, int nsys, int method, double lam, double b1, double b2,
+                       size_t Nmax, hipStream_t st) {
+    const int g = (int)std::min<size_t>((Nmax + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_momentum_b, dim3(g, nsys), dim3(256), 0, st, d, method, lam, b1, b2);
+    PF_HIP(hipGetLastError());
 }
 
 void launch_mix_b(const SysTask* d, const StepTask* s, int nsys, size_t Nmax, hipStream_t st) {

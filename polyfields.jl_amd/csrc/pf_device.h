@@ -201,6 +201,22 @@ __device__ __forceinline__ double ksq_at(const double* G, double jx, double jy, 
 // count over PF_MAXBLK entries, whose loads all go out at once (a loop over begin[] would
 // make each load wait for the previous compare, a chain of dependent scalar loads at the
 // start of every launch).
+// Momentum family update of point i (src/updaters/momentum.jl:86-139); `method`: 0 MOM,
+// 1 RMS, 2 ADAM
+__device__ __forceinline__ void momentum_point(int method, double* mt, double* vt, double* step,
+                                               const double* __restrict__ r, double b1,
+                                               double b2, size_t i) {
+    const double ri = r[i];
+    if (method == 0 || method == 2) mt[i] = b1 * mt[i] + (1 - b1) * ri;
+    if (method == 1 || method == 2) vt[i] = b2 * vt[i] + (1 - b2) * (ri * ri);
+    if (method == 0)
+        step[i] = mt[i];
+    else if (method == 1)
+        step[i] = ri / (sqrt(vt[i]) + 1e-8);
+    else
+        step[i] = mt[i] / (sqrt(vt[i]) + 1e-8);
+}
+
 template <typename T>
 __device__ __forceinline__ int step_block(const Lane<T>* L, int t, int* s_in) {
     // every field is read unconditionally (a short-circuit test would make the loads

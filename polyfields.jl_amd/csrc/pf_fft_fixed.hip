@@ -133,8 +133,9 @@ template <typename T, int NX, int NY, int MODE, int NT, int MINW = 1>
 __global__ void __launch_bounds__(NT, MINW) k_slab_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
     constexpr int M = NX / 2, LD = M + 1, PLANE = NX * NY, KPLANE = LD * NY;
     // ping-pong transforms (one barrier per pass) where a second plane of LDS is cheap
-    constexpr bool PPS = sizeof(T) == 4 && NX <= 64 && NY > 1 && cplan(NX / 2).npass % 2 == 0 &&
-                         cplan(NY).npass % 2 == 0;
+    // (FP32 128 x 128: two planes of 66.5 KB, so only in the one-plane-per-CU form)
+    constexpr bool PPS = sizeof(T) == 4 && (NX <= 64 || (NX <= 128 && MINW == 1)) && NY > 1 &&
+                         cplan(NX / 2).npass % 2 == 0 && cplan(NY).npass % 2 == 0;
     __shared__ cpx<T> S[KPLANE];
     __shared__ cpx<T> S2[PPS ? KPLANE : 1];
     __shared__ cpx<T> TWX[NX];

@@ -553,17 +553,8 @@ void launch_anderson_mix(int h, double* w, const double* r, const HistPtrs& fh,
 __global__ void k_momentum(int method, double* mt, double* vt, double* step,
                            const double* __restrict__ r, double b1, double b2, size_t N) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const double ri = r[i];
-        if (method == 0 || method == 2) mt[i] = b1 * mt[i] + (1 - b1) * ri;
-        if (method == 1 || method == 2) vt[i] = b2 * vt[i] + (1 - b2) * (ri * ri);
-        if (method == 0)
-            step[i] = mt[i];
-        else if (method == 1)
-            step[i] = ri / (sqrt(vt[i]) + 1e-8);
-        else
-            step[i] = mt[i] / (sqrt(vt[i]) + 1e-8);
-    }
+         i += (size_t)gridDim.x * blockDim.x)
+        momentum_point(method, mt, vt, step, r, b1, b2, i);
 }
 
 void launch_momentum(int method, double* mt, double* vt, double* step, const double* r,

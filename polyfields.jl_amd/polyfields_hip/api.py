@@ -542,6 +542,9 @@ class MomentumUpdater(_Updater):
         m = {"MOM": 0, "RMS": 1, "ADAM": 2, "NAG": 3}
         if method not in m:
             raise ValueError("Invalid `method` parameter for MomentumUpdater.")
+        self.method, self.method_id = method, m[method]
+        self.lam, self.gamma = float(lam), float(gamma)
+        self.beta1, self.beta2 = float(beta1), float(beta2)
         h = C.c_void_p()
         call("pf_updater_momentum", C.byref(h), m[method], float(lam), float(gamma),
              float(beta1), float(beta2))
@@ -708,17 +711,30 @@ class Batch:
             s.precision = precision
 
     def scft(self, nhist=10, iter_euler=1, lam_euler=0.01, nsteps=100, ftol=1e-5, mask=None,
-             domain=None, stol=1e-5):
+             domain=None, stol=1e-5, updater=None):
         """scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler); domain) for every
         system, or for those with a true ``mask`` entry; returns one SCFTResults per system
         (ferr NaN: diverged, or not run).  ``domain``: a DomainUpdater whose settings each
         system's own variable-cell updater takes (the object itself is not set up), or None
-        for a fixed cell; ``stol`` as in SCFTOptions."""
+        for a fixed cell; ``stol`` as in SCFTOptions.  ``updater``: an AndersonUpdater or
+        a MomentumUpdater (MOM, RMS or ADAM) whose settings every system's updater takes,
+        in place of the Anderson arguments."""
         res = (_CR * len(self.systems))()
         m = None
         if mask is not None:
             m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
             assert m.shape == (len(self.systems),)
+        if isinstance(updater, AndersonUpdater):
+            nhist, iter_euler, lam_euler = updater.nhist, updater.iter_euler, updater.lam_euler
+        elif isinstance(updater, MomentumUpdater):
+            call("pf_batch_scft_momentum", self._h, updater.method_id, updater.lam,
+                 updater.gamma, updater.beta1, updater.beta2,
+                 domain._h if domain is not None else None, int(nsteps), float(ftol),
+                 float(stol), iptr(m) if m is not None else None, res)
+            return [SCFTResults(r.time, r.nsteps, bool(r.converged), r.ferr, r.serr, r.fhelm)
+                    for r in res]
+        elif updater is not None:
+            raise TypeError("batches run AndersonUpdater or MomentumUpdater")
         if domain is not None:
             call("pf_batch_scft_domain", self._h, int(nhist), int(iter_euler),
                  float(lam_euler), domain._h, int(nsteps), float(ftol), float(stol),

@@ -1,0 +1,97 @@
+"""Batched SCFT with the momentum updaters (pf_batch_scft_momentum): every system follows
+scft!(sys, MomentumUpdater(; method, lam, ...); domain) as the one-at-a-time path runs it
+(src/updaters/momentum.jl:47-139, src/simulate.jl:60-148).
+
+Fixed cell: 2-D hexagonal melts of different grids with :MOM, :RMS and :ADAM.  Variable
+cell: the 64^3 workload's composition and seed protocol on a 16^3 grid (tests/config2.py:
+:MOM with lam = 0.1 and DomainUpdater(nskip = 100, nper = 10)), three seeds in one batch.
+The batch's reductions differ from the one-system path in the last bits, and the
+DomainUpdater's forward-difference Jacobian (dx = 1e-8) scales such differences by about
+1e8, hence the looser bounds with a variable cell.
+"""
+import numpy as np
+import pytest
+
+from config2 import A_CELL, PROTO, build, seeded_fields
+
+P = pytest.importorskip("polyfields_hip")
+pytestmark = pytest.mark.gpu
+
+
+def _hex(n, a, NA, chiN, Ns, seed):
+    cell = P.UnitCell(2, "hexagonal", a)
+    A, B = P.Monomer(1), P.Monomer(2)
+    ch = P.Multiblock([A, B], [NA, 100 - NA], [1.0, 1.0], Ns)
+    itx = P.FloryInteraction()
+    itx.set_interaction(1, 2, chiN / 100.0)
+    s = P.FieldSystem((n, n, 1), cell, monomers=[A, B], precision="fp64")
+    s.add_species(ch, 1.0)
+    s.add_interaction(itx)
+    P.fieldinit(s, seed=seed, scale=0.1)
+    return s
+
+
+HEX = [(32, 4.5, 30, 14.0, 64), (48, 5.5, 35, 18.0, 80), (64, 6.5, 40, 22.0, 100)]
+
+
+@pytest.mark.parametrize("method,lam", [("MOM", 0.1), ("RMS", 0.01), ("ADAM", 0.01)])
+def test_batch_momentum_fixed_cell_matches_one_at_a_time_fp64(method, lam):
+    def make():
+        return [_hex(*c, seed=200 + k) for k, c in enumerate(HEX)]
+
+    single = make()
+    ref = [P.scft(s, P.MomentumUpdater(method=method, lam=lam),
+                  opts=P.SCFTOptions(nsteps=80, ftol=1e-5)) for s in single]
+    batch_sys = make()
+    res = P.Batch(batch_sys).scft(updater=P.MomentumUpdater(method=method, lam=lam),
+                                  nsteps=80, ftol=1e-5)
+    for k, (r, q) in enumerate(zip(res, ref)):
+        print(f"{method} case {k}: batch {r.nsteps} {r.converged} {r.fhelm:.14g} {r.ferr:.6g}; "
+              f"single {q.nsteps} {q.converged} {q.fhelm:.14g} {q.ferr:.6g}", flush=True)
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
+        assert r.ferr == pytest.approx(q.ferr, rel=1e-9)
+        for m in (1, 2):
+            np.testing.assert_allclose(np.asarray(batch_sys[k].fields[m]),
+                                       np.asarray(single[k].fields[m]), rtol=0, atol=1e-10)
+
+
+def _cfg2_16(seed):
+    s, cell, ch = build(P, grid=16, a=A_CELL, precision="fp64")
+    for m, w in seeded_fields(s.shape, seed).items():
+        s.fields[m] = w
+    return s
+
+
+def test_batch_momentum_variable_cell_matches_one_at_a_time_fp64():
+    seeds = [2024, 7, 11]
+    kw = dict(nsteps=400, ftol=PROTO["ftol"])
+    single = [_cfg2_16(sd) for sd in seeds]
+    ref = []
+    for s in single:
+        r = P.scft(s, P.MomentumUpdater(method="MOM", lam=PROTO["lam"]),
+                   domain=P.DomainUpdater(nskip=PROTO["nskip"], nper=PROTO["nper"]),
+                   opts=P.SCFTOptions(stol=PROTO["stol"], **kw))
+        ref.append((r, float(s.cell.params[0])))
+    batch_sys = [_cfg2_16(sd) for sd in seeds]
+    res = P.Batch(batch_sys).scft(updater=P.MomentumUpdater(method="MOM", lam=PROTO["lam"]),
+                                  domain=P.DomainUpdater(nskip=PROTO["nskip"],
+                                                         nper=PROTO["nper"]),
+                                  stol=PROTO["stol"], **kw)
+    for k, (r, (q, qa)) in enumerate(zip(res, ref)):
+        a = float(batch_sys[k].cell.params[0])
+        print(f"seed {seeds[k]}: batch {r.nsteps} {r.converged} fhelm {r.fhelm:.12g} a {a:.10g} "
+              f"serr {r.serr:.3g}; single {q.nsteps} {q.converged} fhelm {q.fhelm:.12g} "
+              f"a {qa:.10g} serr {q.serr:.3g}", flush=True)
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-8)
+        assert a == pytest.approx(qa, rel=1e-6)
+        assert a != pytest.approx(A_CELL, rel=1e-3)   # the cell moved
+
+
+def test_batch_refuses_nag():
+    s = _hex(*HEX[0], seed=1)
+    with pytest.raises(P.PolyFieldsError, match="NAG"):
+        P.Batch([s]).scft(updater=P.MomentumUpdater(method="NAG", lam=0.1), nsteps=5)
