@@ -1040,6 +1040,17 @@ class BatchEngine : public BatchEngineBase {
     LwTask<T>* d_lwt = nullptr;
     std::vector<FillTask<T>> fillt;
     FillTask<T>* d_fillt = nullptr;
+    struct PointTask {
+        const double* w;
+        double vol;
+        size_t n;
+        double* rho_sp;
+        double* partial;
+        int nparts;
+        DevScalars* sc;
+        int isp;
+    };
+    std::vector<PointTask> points;
     double* partial = nullptr;
     size_t partial_len = 0;
     std::vector<int> cached;
@@ -1102,6 +1113,7 @@ class BatchEngine : public BatchEngineBase {
         tasks.clear();
         lwt.clear();
         fillt.clear();
+        points.clear();
         nmax = 0;
         for (int i : act) {
             System& s = *B->sys[i];
@@ -1131,8 +1143,13 @@ class BatchEngine : public BatchEngineBase {
             for (const auto& tk : e.host_tasks()) tasks.push_back(tk);
             for (size_t k = 0; k < s.species.size(); ++k) {
                 const Species& sp = s.species[k];
-                if (sp.kind != 0)
-                    throw Error(-2, "point species are not supported inside a batch");
+                if (sp.kind != 0) {
+                    // point species (src/species/point.jl:36-53): Q and density per system
+                    const int slot = sp.slots[0];
+                    points.push_back({s.w + (size_t)slot * s.n, s.mon_vol[slot], s.n, sp.rho_sp,
+                                      s.partial, reduce_parts(s.n), s.sc, (int)k});
+                    continue;
+                }
                 for (int b = 0; b < sp.nblk(); ++b) {
                     const int slot = sp.block_slot[b];
                     lwt.push_back({e.lw_table((int)k, b), s.w + (size_t)slot * s.n,
@@ -1306,10 +1323,11 @@ class BatchEngine : public BatchEngineBase {
             prev = gr.done;
         }
         const int nparts = reduce_parts(nmax);
-        if (!tasks.empty()) {
+        if (!tasks.empty())
             launch_species_Q<T>(d_tasks, (int)tasks.size(), nmax, partial, nparts, st);
-            launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
-        }
+        for (const PointTask& p : points)
+            launch_point_density(p.w, p.vol, p.n, p.rho_sp, p.partial, p.nparts, p.sc, p.isp, st);
+        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
     }
 };
 

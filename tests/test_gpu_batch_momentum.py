@@ -5,6 +5,7 @@ scft!(sys, MomentumUpdater(; method, lam, ...); domain) as the one-at-a-time pat
 Fixed cell: 2-D hexagonal melts of different grids with :MOM, :RMS and :ADAM.  Variable
 cell: the 64^3 workload's composition and seed protocol on a 16^3 grid (tests/config2.py:
 :MOM with lam = 0.1 and DomainUpdater(nskip = 100, nper = 10)), three seeds in one batch.
+Point species: blends of a point solvent and two homopolymers (the examples.ipynb blend).
 The batch's reductions differ from the one-system path in the last bits, and the
 DomainUpdater's forward-difference Jacobian (dx = 1e-8) scales such differences by about
 1e8, hence the looser bounds with a variable cell.
@@ -95,3 +96,42 @@ def test_batch_refuses_nag():
     s = _hex(*HEX[0], seed=1)
     with pytest.raises(P.PolyFieldsError, match="NAG"):
         P.Batch([s]).scft(updater=P.MomentumUpdater(method="NAG", lam=0.1), nsteps=5)
+
+
+def _blend(L, chi_ab, seed):
+    # point solvent + two homopolymers with v = 2 and a chi triangle (the examples.ipynb
+    # blend), 1-D
+    cell = P.UnitCell(1, "lamellar", L)
+    a, b, c = P.Monomer(1, vol=2.0), P.Monomer(2, vol=2.0), P.Monomer(3, vol=1.0)
+    s = P.FieldSystem((128, 1, 1), cell, monomers=[a, b, c], precision="fp64")
+    s.add_species(P.Point(c), 0.4)
+    s.add_species(P.Homopolymer(a, 100, 1.0), 0.3)
+    s.add_species(P.Homopolymer(b, 100, 1.0), 0.3)
+    itx = P.FloryInteraction()
+    itx.set_interaction(1, 2, chi_ab)
+    itx.set_interaction(1, 3, 1.25)
+    itx.set_interaction(2, 3, 1.25)
+    s.add_interaction(itx)
+    rng = np.random.default_rng(seed)
+    for m in (1, 2, 3):
+        s.fields[m] = 0.05 * rng.standard_normal((1, 1, 128))
+    return s
+
+
+def test_batch_blends_with_point_species_match_one_at_a_time_fp64():
+    cases = [(20.0, 0.5, 1), (24.0, 0.6, 2), (18.0, 0.45, 3)]
+    single = [_blend(*c) for c in cases]
+    ref = [P.scft(s, P.MomentumUpdater(method="MOM", lam=0.01),
+                  opts=P.SCFTOptions(nsteps=300, ftol=1e-6)) for s in single]
+    batch_sys = [_blend(*c) for c in cases]
+    res = P.Batch(batch_sys).scft(updater=P.MomentumUpdater(method="MOM", lam=0.01),
+                                  nsteps=300, ftol=1e-6)
+    for k, (r, q) in enumerate(zip(res, ref)):
+        print(f"blend {k}: batch {r.nsteps} {r.converged} {r.fhelm:.14g}; single {q.nsteps} "
+              f"{q.converged} {q.fhelm:.14g}", flush=True)
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
+        for m in (1, 2, 3):
+            np.testing.assert_allclose(np.asarray(batch_sys[k].density[m]),
+                                       np.asarray(single[k].density[m]), rtol=0, atol=1e-11)
