@@ -1117,6 +1117,9 @@ class BatchEngine : public BatchEngineBase {
         nmax = 0;
         for (int i : act) {
             System& s = *B->sys[i];
+            // pf_batch_create refuses RQM4 systems; a system switched to RQM4 afterwards
+            // is refused here rather than swept with the RK2 kernels
+            if (s.mde != 0) throw Error(-2, "batches run the RK2 propagator only");
             Engine<T>& e = E(i);
             e.prepare();
             nmax = std::max(nmax, s.n);
