@@ -168,15 +168,47 @@ end
 
 mutable struct HIPUpdater
     handle::Ptr{Cvoid}
+    kind::Symbol          # :anderson, :euler or :momentum
+    params::NamedTuple    # the constructor's settings (batches read them)
+end
+
+function _updater(h::Ptr{Cvoid}, kind::Symbol, params::NamedTuple)
+    u = HIPUpdater(h, kind, params)
+    finalizer(x -> ccall((:pf_updater_destroy, LIB), Cint, (Ptr{Cvoid},), x.handle), u)
+    return u
 end
 
 function AndersonUpdater(; nhist::Integer = 50, iter_euler::Integer = 1, lam_euler::Real = 0.01)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     check(ccall((:pf_updater_anderson, LIB), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint, Float64), h,
                 nhist, iter_euler, lam_euler))
-    u = HIPUpdater(h[])
-    finalizer(x -> ccall((:pf_updater_destroy, LIB), Cint, (Ptr{Cvoid},), x.handle), u)
-    return u
+    return _updater(h[], :anderson, (nhist = nhist, iter_euler = iter_euler, lam_euler = lam_euler))
+end
+
+"""`EulerUpdater(; method = :PECE, lam = 0.05)` (src/updaters/euler.jl:20-28)."""
+function EulerUpdater(; method::Symbol = :PECE, lam::Real = 0.05)
+    method in (:PEC, :PECE) ||
+        throw(ArgumentError("Invalid `method` parameter for EulerUpdater. Options include: (:PEC, :PECE)"))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pf_updater_euler, LIB), Cint, (Ref{Ptr{Cvoid}}, Cint, Float64), h,
+                method == :PECE ? 1 : 0, lam))
+    return _updater(h[], :euler, (method = method, lam = lam))
+end
+
+const PF_MOMENTUM = Dict(:MOM => Cint(0), :RMS => Cint(1), :ADAM => Cint(2), :NAG => Cint(3))
+
+"""`MomentumUpdater(; method = :MOM, lam = 0.01, gamma = 0.9, beta1 = 0.9, beta2 = 0.999)`
+(src/updaters/momentum.jl:30-41)."""
+function MomentumUpdater(; method::Symbol = :MOM, lam::Real = 0.01, gamma::Real = 0.9,
+                         beta1::Real = 0.9, beta2::Real = 0.999)
+    haskey(PF_MOMENTUM, method) ||
+        throw(ArgumentError("Invalid `method` parameter for MomentumUpdater."))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:pf_updater_momentum, LIB), Cint,
+                (Ref{Ptr{Cvoid}}, Cint, Float64, Float64, Float64, Float64), h,
+                PF_MOMENTUM[method], lam, gamma, beta1, beta2))
+    return _updater(h[], :momentum,
+                    (method = method, lam = lam, gamma = gamma, beta1 = beta1, beta2 = beta2))
 end
 
 setup!(u::HIPUpdater, sys::HIPSystem) =
@@ -264,12 +296,34 @@ end
 """
     scft!(batch, nhist = 10; iter_euler = 1, lam_euler = 0.01, domain = nothing,
           opts = SCFTOptions(), mask = nothing) -> Vector{SCFTResults}
+    scft!(batch, updater::HIPUpdater; domain = nothing, opts = SCFTOptions(),
+          mask = nothing) -> Vector{SCFTResults}
 
 `scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler); domain, opts)` for every
 system of the batch in lockstep (`pf_batch_scft`, or `pf_batch_scft_domain` with a
-DomainUpdater, whose settings each system's own cell updater takes).  A diverged system
-is reported with ferr = NaN instead of raising.
+DomainUpdater, whose settings each system's own cell updater takes).  The second form
+takes the settings of an AndersonUpdater or a MomentumUpdater (:MOM, :RMS, :ADAM;
+`pf_batch_scft_momentum`).  A diverged system is reported with ferr = NaN instead of
+raising.
 """
+function scft!(b::HIPBatch, u::HIPUpdater; domain = nothing,
+               opts::SCFTOptions = SCFTOptions(), mask = nothing)
+    p = u.params
+    u.kind === :anderson && return scft!(b, p.nhist; iter_euler = p.iter_euler,
+                                         lam_euler = p.lam_euler, domain = domain, opts = opts,
+                                         mask = mask)
+    u.kind === :momentum || throw(ArgumentError("batches run AndersonUpdater or MomentumUpdater"))
+    r = Vector{PFResults}(undef, length(b.systems))
+    m = mask === nothing ? C_NULL : Cint.(mask)
+    dh = domain === nothing ? C_NULL : domain.handle
+    check(ccall((:pf_batch_scft_momentum, LIB), Cint,
+                (Ptr{Cvoid}, Cint, Float64, Float64, Float64, Float64, Ptr{Cvoid}, Cint, Float64,
+                 Float64, Ptr{Cint}, Ptr{PFResults}),
+                b.handle, PF_MOMENTUM[p.method], p.lam, p.gamma, p.beta1, p.beta2, dh,
+                opts.nsteps, opts.ftol, opts.stol, m, r))
+    return [SCFTResults(x.time, x.nsteps, x.converged != 0, x.ferr, x.serr, x.fhelm) for x in r]
+end
+
 function scft!(b::HIPBatch, nhist::Integer = 10; iter_euler::Integer = 1,
                lam_euler::Real = 0.01, domain = nothing, opts::SCFTOptions = SCFTOptions(),
                mask = nothing)
