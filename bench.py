@@ -578,14 +578,12 @@ VC_PH2 = dict(nhist=20, iter_euler=1, lam_euler=0.01, nsteps=1500, ftol=1e-6)
 VC_STOL = 1e-5
 
 
-def sweep_vcell_workload(P, args):
-    """A phase-diagram sweep with the cell relaxed: `--sweep-instances` of the batched
-    sweep's 2-D hexagonal instances (default 64 here), FP64, Euler mixing to ferr < 1e-2
-    on the fixed cell, then AndersonUpdater(nhist=20) with DomainUpdater(nper=5,
-    nskip=50) to ferr < 1e-6 and serr < 1e-5; batched (each DomainUpdater round of all
-    due systems in lockstep) against the same protocol one instance at a time on
-    `--plain-sample` of them."""
-    count = args.sweep_instances if args.sweep_instances != 256 else 64
+def vcell_sweep(P, count, plain_sample):
+    """A phase-diagram sweep with the cell relaxed: `count` of the batched sweep's 2-D
+    hexagonal instances, FP64, Euler mixing to ferr < 1e-2 on the fixed cell, then
+    AndersonUpdater(nhist=20) with DomainUpdater(nper=5, nskip=50) to ferr < 1e-6 and
+    serr < 1e-5; batched (each DomainUpdater round of all due systems in lockstep)
+    against the same protocol one instance at a time on `plain_sample` of them."""
     cfg = sweep_instances(count)
     systems = [sweep_system(P, c, "fp64") for c in cfg]
     b = P.Batch(systems)
@@ -599,7 +597,7 @@ def sweep_vcell_workload(P, args):
     moved = sum(1 for i in solved if abs(cells[i] - cfg[i][4]) > 1e-3 * cfg[i][4])
     del b, systems
     t_plain, n_plain, errs = 0.0, 0, []
-    for i, c in enumerate(cfg[:args.plain_sample]):
+    for i, c in enumerate(cfg[:plain_sample]):
         s = sweep_system(P, c, "fp64")
         t0 = time.perf_counter()
         ok = False
@@ -622,20 +620,28 @@ def sweep_vcell_workload(P, args):
                              "nsteps_batch": r2[i].nsteps, "nsteps_plain": q2.nsteps})
     rate = len(solved) / t_batch
     plain = n_plain / t_plain if t_plain > 0 else 0.0
+    return {"instances": count, "solved": len(solved), "cells_moved": moved,
+            "seconds": t_batch, "batched_solves_per_s": rate, "plain_sample": plain_sample,
+            "plain_solved": n_plain, "plain_seconds": t_plain, "plain_solves_per_s": plain,
+            "speedup_vs_plain": rate / plain if plain > 0 else None, "batch_vs_plain": errs,
+            "protocol": "Euler lam=0.05 to ferr<1e-2 (fixed cell), then Anderson nhist=20 + "
+                        "DomainUpdater(nper=5, nskip=50) to ferr<1e-6, serr<1e-5; FP64"}
+
+
+def sweep_vcell_workload(P, args):
+    """vcell_sweep as its own JSON line: `--sweep-instances` instances (default 64 here),
+    `--plain-sample` of them one at a time."""
+    count = args.sweep_instances if args.sweep_instances != 256 else 64
+    v = vcell_sweep(P, count, args.plain_sample)
     return {"metric": "batched variable-cell SCFT solves/s (2-D hexagonal sweep, FP64)",
-            "value": rate, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-            "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded 0.1*N(0,1))",
+            "value": v["batched_solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1,
+            "warmup": 0, "ms_per_step": v["seconds"] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded 0.1*N(0,1))",
             "config": {"workload": f"{count} hexagonal diblock instances of the batched "
-                                   "sweep (grids 32^2-96^2), Euler lam=0.05 to ferr<1e-2 "
-                                   "(fixed cell), then Anderson nhist=20 + DomainUpdater("
-                                   "nper=5, nskip=50) to ferr<1e-6, serr<1e-5",
+                                   "sweep (grids 32^2-96^2), " + v["protocol"],
                        "instances": count},
-            "solved": len(solved), "cells_moved": moved, "seconds": t_batch,
-            "plain_sample": args.plain_sample, "plain_solved": n_plain,
-            "plain_seconds": t_plain, "plain_solves_per_s": plain,
-            "speedup_vs_plain": rate / plain if plain > 0 else None,
-            "batch_vs_plain": errs}
+            **{k: x for k, x in v.items() if k != "protocol"}}
 
 
 def copy_workload(P, args):
@@ -800,6 +806,8 @@ def main():
         line["single_instance_steps_per_s"] = single_instance(P, args, nsteps_mde)
     if not args.no_sweep:
         line["batched_scft"] = batched_scft(P, args.sweep_instances, args.plain_sample)
+        # the same kind of sweep with the cell relaxed (64 instances, FP64, 4 one at a time)
+        line["batched_vcell_scft"] = vcell_sweep(P, 64, 4)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.seed)
     print(json.dumps(line))
