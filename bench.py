@@ -89,10 +89,31 @@ def cpu_baseline(seed):
         so.density_update()
         ts.append(time.perf_counter() - t0)
     t = statistics.median(ts)
-    return {"value": 2 * ch.Ns / t, "unit": "MDE steps/s", "cores": cores, "kind": "port",
-            "sample": f"median of 3 FP64 density! evaluations of one 64^3 instance "
-                      f"({2 * ch.Ns} MDE steps each) after 1 warm-up; CPU oracle "
-                      f"(scipy.fft workers={cores}, pointwise single-threaded)"}
+    state = {"Q": ch.Q, "fhelm": so.free_energy(),
+             "rho": {m: np.asarray(so.density[m], dtype=float) for m in so.mids}}
+    return ({"value": 2 * ch.Ns / t, "unit": "MDE steps/s", "cores": cores, "kind": "port",
+             "sample": f"median of 3 FP64 density! evaluations of one 64^3 instance "
+                       f"({2 * ch.Ns} MDE steps each) after 1 warm-up; CPU oracle "
+                       f"(scipy.fft workers={cores}, pointwise single-threaded)"}, state)
+
+
+def gpu_vs_oracle(P, seed, state):
+    """The GPU's density!, Q and fhelm of the headline's first instance in FP64 and FP32
+    against the oracle's evaluation of the same fields (the CPU baseline's run)."""
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s, ch = build(P, prec, seed)
+        for m, w in seeded_fields(s.shape, seed).items():
+            s.fields[m] = w
+        s.density_update()
+        q = s.species_scalars(0)[0]
+        dr = max(float(np.abs(np.asarray(s.density[m]) - state["rho"][m]).max() /
+                       np.abs(state["rho"][m]).max()) for m in s.mids)
+        out[prec] = {"Q_rel": abs(q - state["Q"]) / abs(state["Q"]),
+                     "rho_max_rel": dr, "fhelm_abs": abs(s.free_energy() - state["fhelm"])}
+        del s
+    out["workload"] = "instance 0 of the headline batch (64^3, seeded fields), one density!"
+    return out
 
 
 # ---------------------------------------------------------------------------------
@@ -809,7 +830,8 @@ def main():
         # the same kind of sweep with the cell relaxed (64 instances, FP64, 4 one at a time)
         line["batched_vcell_scft"] = vcell_sweep(P, 64, 4)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.seed)
+        line["cpu_baseline"], state = cpu_baseline(args.seed)
+        line["gpu_vs_oracle"] = gpu_vs_oracle(P, args.seed, state)
     print(json.dumps(line))
 
 
