@@ -755,6 +755,13 @@ def main():
     for _ in range(args.steps):
         batch.density_update()
     dt = time.perf_counter() - t0
+    # the chain of propagate! calls alone (the sweeps of density!, without Q, Simpson and
+    # pooling), timed the same way: SURVEY's isolated step chain
+    batch.sweeps()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.sweeps()
+    dt_chain = time.perf_counter() - t0
     # kernel-level figures from a separate pass: HIP events on the launching streams
     # bracket one launch in 8, run 4 times back to back (the events' overhead spread over
     # the repeats); the launches run eagerly and one stream at a time, so each timed
@@ -814,6 +821,15 @@ def main():
                      "bytes_per_launch": dom[1], "avg_launch_us": dom[2] * 1e6},
         "step_roofline_frac": value * b_step / HBM_PEAK,
         "kernel_avg_us": {"slab": slab_avg * 1e6, "pencil": pen_avg * 1e6},
+        # the propagator sweeps alone: MDE steps/s and their fraction of HBM peak on the
+        # same B_step (density! adds Q, the Simpson pass and pooling to this)
+        "step_chain": {"value": nsteps_mde * args.instances * args.steps / dt_chain,
+                       "ms_per_step": dt_chain * 1e3 / args.steps,
+                       "step_roofline_frac": nsteps_mde * args.instances * args.steps /
+                       dt_chain * b_step / HBM_PEAK,
+                       "what": "pf_batch_sweeps: the q and q+ sweeps of every instance "
+                               "(exponential tables, xy-slab and z-pencil launches), "
+                               "without Q, Simpson and pooling"},
     }
     if args.precision == "fp32":
         # the timing pass repeats launches, so the batch's densities are recomputed first

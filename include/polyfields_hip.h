@@ -260,6 +260,12 @@ int pf_batch_destroy(pf_batch* b);
 int pf_batch_density(pf_batch* b);
 int pf_batch_set_precision(pf_batch* b, int precision);
 int pf_batch_residuals(pf_batch* b);
+/* The propagator sweeps of density! alone (src/species/multiblock.jl:136-172): the
+   exponential tables from the current fields and the q / q+ sweeps of every chain of
+   every system, leaving the stacks filled (readable with pf_system_get_propagator); no
+   Q, Simpson, densities or pooling.  The chain of propagate! calls as one launch
+   sequence, replayed from a graph like pf_batch_density. */
+int pf_batch_sweeps(pf_batch* b);
 /* Graph replay of the density!/residuals! launch sequence (default on): the launches of
    every lane group, Q, Simpson, pooling and residuals are captured once per active set
    and replayed; off runs them eagerly. */

@@ -655,6 +655,14 @@ int pf_batch_set_precision(pf_batch* b, int precision) {
     return guard([&] { B(b).set_precision(precision); });
 }
 
+int pf_batch_sweeps(pf_batch* b) {
+    return guard([&] {
+        for (auto* s : B(b).sys) s->sync();
+        B(b).sweeps(b->all);
+        PF_HIP(hipStreamSynchronize(B(b).stream));
+    });
+}
+
 int pf_batch_residuals(pf_batch* b) {
     return guard([&] {
         for (auto* s : B(b).sys) s->sync();

@@ -219,6 +219,10 @@ void Batch::run(int kind, const std::vector<int>& act) {
     size_t nmax = 0;
     for (int i : act) nmax = std::max(nmax, sys[i]->n);
     auto seq = [&] {
+        if (kind == 2) {
+            eng->launch_sweeps(act);
+            return;
+        }
         eng->launch(act);
         launch_muphi_pool_b(d_act, (int)act.size(), nmax, stream);
         if (kind == 1)
@@ -255,6 +259,8 @@ void Batch::run(int kind, const std::vector<int>& act) {
 void Batch::density(const std::vector<int>& act) { run(0, act); }
 
 void Batch::residuals(const std::vector<int>& act) { run(1, act); }
+
+void Batch::sweeps(const std::vector<int>& act) { run(2, act); }
 
 void Batch::sums_launch(const std::vector<int>& act) {
     bind_active(act);

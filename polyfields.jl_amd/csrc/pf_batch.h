@@ -87,6 +87,8 @@ class BatchEngineBase {
     // the launch sequence of density! for a prepared active set (stream-ordered only, so
     // it can be captured into a graph)
     virtual void launch(const std::vector<int>& active) = 0;
+    // its first part: the exponential tables and the q / q+ sweeps of every lane group
+    virtual void launch_sweeps(const std::vector<int>& active) = 0;
     virtual long generation() const = 0;   // bumped by every rebuild
     virtual void invalidate() = 0;          // force a rebuild (schedule change)
     virtual bool timing_on() const = 0;
@@ -176,9 +178,9 @@ class Batch {
         std::vector<int> act;
         long gen = -1;
         const SysTask* d_act = nullptr;
-    } graphs[2];
+    } graphs[3];
     void drop_graphs();
-    void run(int kind, const std::vector<int>& act);   // 0 density, 1 residuals
+    void run(int kind, const std::vector<int>& act);   // 0 density, 1 residuals, 2 sweeps
 
     Batch(const std::vector<System*>& s);
     ~Batch();
@@ -186,6 +188,8 @@ class Batch {
     void set_precision(int p);
     void density(const std::vector<int>& act);
     void residuals(const std::vector<int>& act);
+    // the propagator sweeps alone (q and q+ of every chain), no Q, Simpson or pooling
+    void sweeps(const std::vector<int>& act);
     void sys_sums(const std::vector<int>& act, std::vector<double>& out);  // 5 per system
     // sys_sums in two halves: queue the reduction and its copy to the host, then wait for
     // that copy only (work queued in between keeps running)

@@ -1289,7 +1289,18 @@ class BatchEngine : public BatchEngineBase {
     void invalidate() override { cached.clear(); }
     bool timing_on() const override { return tim; }
 
-    void launch(const std::vector<int>&) override {
+    void launch(const std::vector<int>& act) override {
+        launch_sweeps(act);
+        hipStream_t st = B->stream;
+        const int nparts = reduce_parts(nmax);
+        if (!tasks.empty())
+            launch_species_Q<T>(d_tasks, (int)tasks.size(), nmax, partial, nparts, st);
+        for (const PointTask& p : points)
+            launch_point_density(p.w, p.vol, p.n, p.rho_sp, p.partial, p.nparts, p.sc, p.isp, st);
+        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
+    }
+
+    void launch_sweeps(const std::vector<int>&) override {
         hipStream_t st = B->stream;
         launch_lw_fill_b<T>(d_lwt, (int)lwt.size(), d_fillt, (int)fillt.size(), nmax, st);
         PF_HIP(hipEventRecord(fork, st));
@@ -1325,12 +1336,6 @@ class BatchEngine : public BatchEngineBase {
             PF_HIP(hipStreamWaitEvent(st, gr.done, 0));
             prev = gr.done;
         }
-        const int nparts = reduce_parts(nmax);
-        if (!tasks.empty())
-            launch_species_Q<T>(d_tasks, (int)tasks.size(), nmax, partial, nparts, st);
-        for (const PointTask& p : points)
-            launch_point_density(p.w, p.vol, p.n, p.rho_sp, p.partial, p.nparts, p.sc, p.isp, st);
-        if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
     }
 };
 

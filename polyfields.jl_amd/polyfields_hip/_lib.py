@@ -89,6 +89,7 @@ SIGNATURES = {
     "pf_batch_destroy": (I, [P]),
     "pf_batch_density": (I, [P]),
     "pf_batch_residuals": (I, [P]),
+    "pf_batch_sweeps": (I, [P]),
     "pf_batch_set_precision": (I, [P, I]),
     "pf_batch_scft": (I, [P, I, I, D, I, D, IP, C.POINTER(ScftResults)]),
     "pf_batch_scft_domain": (I, [P, I, I, D, P, I, D, D, IP, C.POINTER(ScftResults)]),

@@ -704,6 +704,11 @@ class Batch:
     def residuals_update(self):
         call("pf_batch_residuals", self._h)
 
+    def sweeps(self):
+        """The propagator sweeps of density! alone (q and q+ of every chain), no Q,
+        Simpson or densities."""
+        call("pf_batch_sweeps", self._h)
+
     def set_precision(self, precision):
         """Switch every system's propagator between "fp64" and "fp32"."""
         call("pf_batch_set_precision", self._h, {"fp64": 0, "fp32": 1}[precision])

@@ -135,3 +135,20 @@ def test_batch_blends_with_point_species_match_one_at_a_time_fp64():
         for m in (1, 2, 3):
             np.testing.assert_allclose(np.asarray(batch_sys[k].density[m]),
                                        np.asarray(single[k].density[m]), rtol=0, atol=1e-11)
+
+
+def test_batch_sweeps_fill_the_stacks_as_density_does():
+    # pf_batch_sweeps runs the propagate! chain alone: the q and q+ slices equal those of
+    # a full batched density! of the same fields (FP32 64^3 and FP64 2-D)
+    for prec, make in (("fp32", lambda k: build(P, grid=64, a=A_CELL, precision="fp32")[0]),
+                       ("fp64", lambda k: _hex(*HEX[k % 3], seed=300 + k))):
+        a = [make(k) for k in range(3)]
+        b = [make(k) for k in range(3)]
+        P.Batch(a).density_update()
+        P.Batch(b).sweeps()
+        for sa, sb in zip(a, b):
+            ca, cb = sa.species[0], sb.species[0]
+            for s in (0, 1, ca.Ns // 2, ca.Ns):
+                np.testing.assert_array_equal(cb.q[s], ca.q[s])
+                np.testing.assert_array_equal(cb.qc[s], ca.qc[s])
+        print(f"{prec}: sweeps() stacks equal density_update() stacks", flush=True)
