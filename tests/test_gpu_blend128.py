@@ -4,6 +4,8 @@ fraction 0.15); chi_AB N = 25, chi_BC N = 20, chi_AC N = 35 (N = 200); fields
 0.1 N(0,1); cell edges a, b, c drawn uniform in [6, 10] Rg (Rg = sqrt(200/6) b).
 
 * the same composition on a 16x12x8 grid: FP64 GPU against the FP64 oracle;
+* 128^3: the FP64 GPU path against the FP64 oracle at full size (density, Q, free
+  energy, stress; about 20 s of oracle time on the host);
 * 128^3: FP32 propagator / FP64 sums against the FP64 GPU path (density, Q, free energy,
   field error, stress), and the FP64 free-energy gradient against central differences
   (h = 1e-6) in the fields and in the three cell parameters.
@@ -80,6 +82,36 @@ def big():
     for m, w in f.items():
         s32.fields[m] = w
     return s64, s32, f
+
+
+def test_blend_128_fp64_matches_oracle_full_size(big):
+    import os
+    import time
+    s64, _, f = big
+    O.set_fft_workers(max(1, min(16, len(os.sched_getaffinity(0)))))
+    so = blend(O, (128, 128, 128), cell_params())
+    for m, w in f.items():
+        so.fields[m] = w
+    t0 = time.perf_counter()
+    so.density_update()
+    st_o = so.scfstress()
+    t_o = time.perf_counter() - t0
+    for m, w in f.items():
+        s64.fields[m] = w
+    s64.density_update()
+    for isp in range(2):
+        assert s64.species[isp].Q == pytest.approx(so.species[isp].Q, rel=1e-12)
+    e_rho = 0.0
+    for m in so.mids:
+        a, b = np.asarray(s64.density[m]), np.asarray(so.density[m])
+        e_rho = max(e_rho, float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+    assert e_rho < 1e-11
+    assert s64.free_energy() == pytest.approx(so.free_energy(), rel=1e-11)
+    st_g = s64.scfstress()
+    np.testing.assert_allclose(st_g, st_o, rtol=1e-9, atol=1e-15)
+    print(f"128^3 blend FP64 GPU vs oracle: rho {e_rho:.2e} relative, stress "
+          f"{float(np.max(np.abs(st_g - st_o) / np.abs(st_o))):.2e}; oracle {t_o:.1f} s")
+    del so
 
 
 def test_blend_128_fp32_vs_fp64(big):
