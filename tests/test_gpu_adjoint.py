@@ -164,6 +164,33 @@ def cfg4_state():
 
 
 @pytest.mark.parametrize("functional", ["reference", "textbook"])
+def test_cfg4_gradient_matches_oracle_full_size(cfg4_state, functional):
+    # the config-4 system at its full 64^3 grid: the GPU reverse sweep against the
+    # oracle's (about 3 s of oracle time per functional on the host)
+    import os
+    sys, fields = cfg4_state
+    O.set_fft_workers(max(1, min(16, len(os.sched_getaffinity(0)))))
+    so = diblock(O, (64, 64, 64), 3, "cubic", (A_RG,), N=100, f=0.35, chi=0.2, Ns=100,
+                 b=(1.0, 1.0))
+    set_fields(so, fields)
+    set_fields(sys, fields)
+    go = OA.fhelm_grad(so, functional)
+    gg = P.fhelm_grad(sys, functional)
+    dwo = np.stack([go.dw[m] for m in so.mids])
+    dwg = np.stack([gg.dw[m] for m in sys.mids])
+    print(f"64^3 gradient ({functional}) GPU vs oracle: fhelm {abs(gg.fhelm / go.fhelm - 1):.2e}, "
+          f"dw {relmax(dwg, dwo):.2e}, da {relmax(gg.dcell, go.dcell):.2e}", flush=True)
+    assert gg.fhelm == pytest.approx(go.fhelm, rel=1e-11, abs=1e-13)
+    assert relmax(dwg, dwo) < 1e-10
+    assert relmax(gg.dcell, go.dcell) < 1e-9
+    for k, v in go.dchi.items():
+        assert gg.dchi[k] == pytest.approx(v, rel=1e-10)
+    for a, b in zip(gg.dN, go.dN):
+        for x, y in zip(a, b):
+            assert x == pytest.approx(y, rel=1e-9, abs=1e-13)
+
+
+@pytest.mark.parametrize("functional", ["reference", "textbook"])
 def test_cfg4_fields_fd_64_directions(cfg4_state, functional):
     sys, fields = cfg4_state
     set_fields(sys, fields)
