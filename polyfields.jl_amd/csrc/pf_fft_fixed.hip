@@ -147,10 +147,12 @@ __global__ void __launch_bounds__(NT, MINW) k_slab_fx(const Lane<T>* __restrict_
     int s_in;
     if constexpr (MODE == SLAB_INVFWD) {
         // Every lane field the first loads need comes in one round of scalar loads (the
-        // empty asm keeps them ahead of every branch).  The LW rows of this plane are
-        // loaded first, so that their latency overlaps the inverse transforms, then the
-        // spectrum plane and the twiddles.  A finished lane (t >= Ns) loads its (valid)
-        // buffers too and returns before its first barrier.
+        // empty asm keeps them ahead of every branch).  The spectrum plane and the
+        // twiddles are loaded first; the LW rows of this plane are issued once they have
+        // arrived, and stream in during the inverse transforms (loads complete in order,
+        // so LW rows issued first would hold up the spectrum's first LDS store).  A
+        // finished lane (t >= Ns) loads its (valid) buffers too and returns before its
+        // first barrier.
         unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf),
                            txr = __builtin_bit_cast(unsigned long long, L->twx),
                            tyr = __builtin_bit_cast(unsigned long long, L->twy);
@@ -166,18 +168,17 @@ __global__ void __launch_bounds__(NT, MINW) k_slab_fx(const Lane<T>* __restrict_
         const auto kb = gp(kbp);
         constexpr int NP = (NX / 2) * NY, PER = (NP + NT - 1) / NT;
         const auto lw = gp(reinterpret_cast<const cpx<T>*>(lwp + (size_t)z * PLANE));
+        lds_copy_in<T, KPLANE, NT>(S, kbp);
+        lds_copy_in<T, NX, NT>(TWX, __builtin_bit_cast(const cpx<T>*, txr));
+        lds_copy_in<T, NY, NT>(TWY, __builtin_bit_cast(const cpx<T>*, tyr));
+        // the LW loads stay behind the copies above (no global access moves across this)
+        asm volatile("" ::: "memory");
         cpx<T> wr[PER];
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int i = k * NT + threadIdx.x;
             if (i < NP) wr[k] = lw[i];
         }
-        lds_copy_in<T, KPLANE, NT>(S, kbp);
-        lds_copy_in<T, NX, NT>(TWX, __builtin_bit_cast(const cpx<T>*, txr));
-        lds_copy_in<T, NY, NT>(TWY, __builtin_bit_cast(const cpx<T>*, tyr));
-        // the LW values count as used here, so their loads stay ahead of the exit test
-#pragma unroll
-        for (int k = 0; k < PER; ++k) asm volatile("" : "+v"(wr[k].x), "+v"(wr[k].y));
         if (t >= nsteps) return;
         __syncthreads();
         if constexpr (PPS)
