@@ -72,7 +72,6 @@ Batch::Batch(const std::vector<System*>& s) : sys(s) {
         if (x->precision != precision) throw Error(-2, "all systems of a batch need one precision");
         if (x->device != sys[0]->device) throw Error(-2, "all systems of a batch need one device");
         if (x->nmon() > PF_MAXMON) throw Error(-2, "too many monomers");
-        if (x->mde != 0) throw Error(-2, "batches run the RK2 propagator only");
     }
     PF_HIP(hipSetDevice(sys[0]->device));
     PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));

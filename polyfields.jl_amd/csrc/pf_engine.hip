@@ -450,6 +450,9 @@ class Engine : public EngineBase {
     T* q_stack(int isp) const { return sb[isp].q; }
     T* qc_stack(int isp) const { return sb[isp].qc; }
     T* lw_table(int isp, int blk) const { return sb[isp].lw[blk]; }
+    T* lw_half_table(int isp, int blk) const {
+        return sb[isp].lw2.empty() ? nullptr : sb[isp].lw2[blk];
+    }
     int tiles() const { return ntiles; }
     int tile_w() const { return tile; }
     int plane_lds() const { return lds_plane; }
@@ -1028,6 +1031,7 @@ class BatchEngine : public BatchEngineBase {
     Batch* B;
     struct Group {
         int nx, ny, nz, off, count, maxNs, ntiles, tile, lds_plane, lds_pencil;
+        int mde;   // 0 RK2, 1 RQM4 (lanes are grouped by grid size and method)
         hipStream_t st = nullptr;
         hipEvent_t done = nullptr;
     };
@@ -1117,20 +1121,20 @@ class BatchEngine : public BatchEngineBase {
         nmax = 0;
         for (int i : act) {
             System& s = *B->sys[i];
-            // pf_batch_create refuses RQM4 systems; a system switched to RQM4 afterwards
-            // is refused here rather than swept with the RK2 kernels
-            if (s.mde != 0) throw Error(-2, "batches run the RK2 propagator only");
             Engine<T>& e = E(i);
             e.prepare();
             nmax = std::max(nmax, s.n);
             int gi = -1;
             for (size_t k = 0; k < g.size(); ++k)
-                if (g[k].nx == s.dims[0] && g[k].ny == s.dims[1] && g[k].nz == s.dims[2]) gi = (int)k;
+                if (g[k].nx == s.dims[0] && g[k].ny == s.dims[1] && g[k].nz == s.dims[2] &&
+                    g[k].mde == s.mde)
+                    gi = (int)k;
             if (gi < 0) {
                 Group ng{};
                 ng.nx = s.dims[0];
                 ng.ny = s.dims[1];
                 ng.nz = s.dims[2];
+                ng.mde = s.mde;
                 ng.ntiles = e.tiles();
                 ng.tile = e.tile_w();
                 ng.lds_plane = e.plane_lds();
@@ -1155,8 +1159,13 @@ class BatchEngine : public BatchEngineBase {
                 }
                 for (int b = 0; b < sp.nblk(); ++b) {
                     const int slot = sp.block_slot[b];
-                    lwt.push_back({e.lw_table((int)k, b), s.w + (size_t)slot * s.n,
-                                   (-(sp.ds_block[b] / 2.0)) * s.mon_vol[slot], s.n});
+                    const double coef = (-(sp.ds_block[b] / 2.0)) * s.mon_vol[slot];
+                    lwt.push_back({e.lw_table((int)k, b), s.w + (size_t)slot * s.n, coef, s.n});
+                    // RQM4 half-step table exp(coef w / 2) (mde.jl:51): coef / 2 scales
+                    // exactly, so the product rounds as the one-system table's does
+                    if (s.mde == 1)
+                        lwt.push_back({e.lw_half_table((int)k, b), s.w + (size_t)slot * s.n,
+                                       coef / 2.0, s.n});
                 }
                 fillt.push_back({e.q_stack((int)k), s.n});
                 if (e.qc_stack((int)k)) fillt.push_back({e.qc_stack((int)k) + (size_t)sp.Ns * s.n, s.n});
@@ -1310,11 +1319,27 @@ class BatchEngine : public BatchEngineBase {
             // launch has the device to itself (the figure a kernel trace reports)
             PF_HIP(hipStreamWaitEvent(gr.st, tim ? prev : fork, 0));
             const Lane<T>* dl = d_lanes + gr.off;
-            if (gr.nz == 1) {
+            if (gr.nz == 1 && gr.mde == 1) {
+                // RQM4 planes: the generic kernel with both planes in LDS (mde.jl:83-104)
+                timed(2, gr.st, gr.count, [&] {
+                    launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, 2 * gr.lds_plane, 256, gr.st);
+                });
+            } else if (gr.nz == 1) {
                 timed(2, gr.st, gr.count, [&] {
                     if (!fixed_planar<T>(dl, gr.count, gr.nx, gr.ny, 0, gr.maxNs, gr.st))
                         launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.lds_plane, 256, gr.st);
                 });
+            } else if (gr.mde == 1) {
+                // RQM4 in 3-D: per contour step the full- and half-step spectra through the
+                // pencils and three slab launches, as the one-system engine runs them
+                launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_START, 0, 2 * gr.lds_plane, gr.st);
+                for (int t = 0; t < gr.maxNs; ++t) {
+                    launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 0);
+                    launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 1);
+                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_MID, t, 2 * gr.lds_plane, gr.st);
+                    launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 1);
+                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_END, t, 2 * gr.lds_plane, gr.st);
+                }
             } else {
                 timed(0, gr.st, gr.count, [&] {
                     if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_FWD, 0, gr.st))

@@ -152,3 +152,41 @@ def test_batch_sweeps_fill_the_stacks_as_density_does():
                 np.testing.assert_array_equal(cb.q[s], ca.q[s])
                 np.testing.assert_array_equal(cb.qc[s], ca.qc[s])
         print(f"{prec}: sweeps() stacks equal density_update() stacks", flush=True)
+
+
+def test_batch_rqm4_matches_one_at_a_time_fp64():
+    # RQM4 systems (mde.jl:83-104) in a batch beside an RK2 one: 1-D, 2-D and 3-D grids,
+    # the densities of one batched density! and an Anderson scft! equal one-at-a-time
+    def make():
+        out = []
+        for k, (dims, dim, cr, prm) in enumerate([((64, 1, 1), 1, "lamellar", (12.0,)),
+                                                  ((32, 32, 1), 2, "hexagonal", (8.0,)),
+                                                  ((16, 16, 16), 3, "cubic", (7.0,))]):
+            cell = P.UnitCell(dim, cr, *prm)
+            A, B = P.Monomer(1), P.Monomer(2)
+            ch = P.Multiblock([A, B], [35, 65], [1.0, 1.0], 40)
+            itx = P.FloryInteraction()
+            itx.set_interaction(1, 2, 0.2)
+            s = P.FieldSystem(dims, cell, monomers=[A, B], precision="fp64", mde="RQM4")
+            s.add_species(ch, 1.0)
+            s.add_interaction(itx)
+            P.fieldinit(s, seed=500 + k, scale=0.1)
+            out.append(s)
+        out.append(_hex(*HEX[0], seed=77))   # an RK2 system in the same batch
+        return out
+
+    single, batch_sys = make(), make()
+    for s in single:
+        s.density_update()
+    P.Batch(batch_sys).density_update()
+    for a, b in zip(single, batch_sys):
+        for m in (1, 2):
+            np.testing.assert_array_equal(np.asarray(b.density[m]), np.asarray(a.density[m]))
+    ref = [P.scft(s, P.AndersonUpdater(nhist=10), opts=P.SCFTOptions(nsteps=40, ftol=1e-7))
+           for s in single]
+    res = P.Batch(batch_sys).scft(nhist=10, nsteps=40, ftol=1e-7)
+    for k, (r, q) in enumerate(zip(res, ref)):
+        print(f"rqm4 case {k}: batch {r.nsteps} {r.fhelm:.14g}; single {q.nsteps} {q.fhelm:.14g}",
+              flush=True)
+        assert r.nsteps == q.nsteps and r.converged == q.converged
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
