@@ -317,14 +317,22 @@ int pf_batch_scft_domain(pf_batch* b, int nhist, int iter_euler, double lam_eule
                          const pf_domain* domain, int nsteps, double ftol, double stol,
                          const int* mask, pf_scft_results* results);
 /* The batched loop with scft!(sys, MomentumUpdater(; method, lam, gamma, beta1, beta2);
-   domain) for every system (src/updaters/momentum.jl:47-139): method 0 :MOM, 1 :RMS,
-   2 :ADAM (3 :NAG is refused: it evaluates its residuals at a look-ahead point, which
-   the batched loop does not schedule).  domain NULL: a fixed cell (stol unused); else
-   as pf_batch_scft_domain.  One kernel applies every active system's moment update and
-   field step.  No counterpart in the reference beyond scft! itself. */
+   domain) for every system (src/updaters/momentum.jl:47-165): method 0 :MOM, 1 :RMS,
+   2 :ADAM, 3 :NAG (residuals! of every active system at its look-ahead fields, inside
+   the step).  domain NULL: a fixed cell (stol unused); else as pf_batch_scft_domain.
+   One kernel applies every active system's moment update and field step.  No
+   counterpart in the reference beyond scft! itself. */
 int pf_batch_scft_momentum(pf_batch* b, int method, double lam, double gamma, double beta1,
                            double beta2, const pf_domain* domain, int nsteps, double ftol,
                            double stol, const int* mask, pf_scft_results* results);
+/* The batched loop with scft!(sys, EulerUpdater(; method, lam); domain) for every system
+   (src/updaters/euler.jl:36-66): method 0 :PEC, 1 :PECE.  Every step runs the predictor
+   of all active systems in one kernel, one batched residuals! at the predicted fields,
+   the corrector, and for PECE a second batched residuals!.  domain as in
+   pf_batch_scft_momentum.  No counterpart in the reference beyond scft! itself. */
+int pf_batch_scft_euler(pf_batch* b, int method, double lam, const pf_domain* domain,
+                        int nsteps, double ftol, double stol, const int* mask,
+                        pf_scft_results* results);
 int pf_batch_timing(pf_batch* b, int on);
 int pf_batch_timing_read(pf_batch* b, int kind, double* total_ms, int* launches);
 /* Mean number of propagator lanes covered by the sampled launches of one kind (3-D

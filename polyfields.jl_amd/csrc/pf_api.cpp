@@ -704,6 +704,18 @@ int pf_batch_scft_momentum(pf_batch* b, int method, double lam, double gamma, do
     });
 }
 
+int pf_batch_scft_euler(pf_batch* b, int method, double lam, const pf_domain* domain,
+                        int nsteps, double ftol, double stol, const int* mask,
+                        pf_scft_results* results) {
+    return guard([&] {
+        need(results, "results");
+        need(!domain || domain->d, "domain");
+        auto r = B(b).scft_euler(method, lam, nsteps, ftol, mask, domain ? domain->d : nullptr,
+                                 stol);
+        copy_batch_results(r, results);
+    });
+}
+
 int pf_batch_scft_domain(pf_batch* b, int nhist, int iter_euler, double lam_euler,
                          const pf_domain* domain, int nsteps, double ftol, double stol,
                          const int* mask, pf_scft_results* results) {

@@ -404,11 +404,14 @@ struct AndersonStepper : BatchStepper {
 // kernel updates the moments and the fields of every active system
 struct MomentumStepper : BatchStepper {
     int method;
-    double lam, b1, b2;
-    MomentumStepper(int m, double l, double bb1, double bb2) : method(m), lam(l), b1(bb1), b2(bb2) {}
+    double lam, gamma, b1, b2;
+    MomentumStepper(int m, double l, double g, double bb1, double bb2)
+        : method(m), lam(l), gamma(g), b1(bb1), b2(bb2) {}
     const char* noncanonical_error() const override {
         return "batched SCFT runs systems in the canonical ensemble only";
     }
+    // NAG evaluates residuals! at its look-ahead fields inside the step
+    bool starts_with_residuals() const override { return method != 3; }
     void setup(Batch& b, const std::vector<int>& run) override {
         b.free_momentum();
         b.mom_state.assign(b.sys.size(), Batch::MomState{});
@@ -418,6 +421,7 @@ struct MomentumStepper : BatchStepper {
             Batch::MomState& m = b.mom_state[i];
             for (double** p : {&m.mt, &m.vt, &m.step})
                 *p = (double*)dev_alloc(sizeof(double) * N);
+            if (method == 3) m.tmp = (double*)dev_alloc(sizeof(double) * N);
             // setup! (momentum.jl:47-61): step = the first residual, zero moments
             PF_HIP(hipMemcpyAsync(m.step, s.res, sizeof(double) * N, hipMemcpyDeviceToDevice,
                                   b.stream));
@@ -428,22 +432,51 @@ struct MomentumStepper : BatchStepper {
     void update(Batch& b, const std::vector<int>& act, std::vector<BatchResult>&) override {
         size_t Nmax = 0;
         for (int i : act) Nmax = std::max(Nmax, b.sys[i]->n * b.sys[i]->nmon());
-        if (act != b.mom_act || !b.d_mom) {
-            std::vector<MomTask> t;
-            for (int i : act) {
-                System& s = *b.sys[i];
-                const Batch::MomState& m = b.mom_state[i];
-                t.push_back({m.mt, m.vt, m.step, s.w, s.res, s.n * s.nmon()});
-            }
-            PF_HIP(hipStreamSynchronize(b.stream));   // queued work may read the old table
-            dev_free(b.d_mom);
-            b.d_mom = (MomTask*)dev_alloc(sizeof(MomTask) * t.size());
-            PF_HIP(hipMemcpyAsync(b.d_mom, t.data(), sizeof(MomTask) * t.size(),
-                                  hipMemcpyHostToDevice, b.stream));
-            PF_HIP(hipStreamSynchronize(b.stream));
-            b.mom_act = act;
+        b.bind_momentum(act);
+        if (method == 3) {
+            // NAG (momentum.jl:141-165): residuals! at w + lam m, the momentum update with
+            // gamma, then the step from the saved fields
+            launch_stage_b(b.d_mom, (int)act.size(), STAGE_NAG_LOOK, lam, 0.0, 0.0, Nmax,
+                           b.stream);
+            b.residuals(act);
+            launch_stage_b(b.d_mom, (int)act.size(), STAGE_NAG_STEP, lam, gamma, b2, Nmax,
+                           b.stream);
+        } else {
+            launch_momentum_b(b.d_mom, (int)act.size(), method, lam, b1, b2, Nmax, b.stream);
         }
-        launch_momentum_b(b.d_mom, (int)act.size(), method, lam, b1, b2, Nmax, b.stream);
+    }
+};
+
+// EulerUpdater (src/updaters/euler.jl:36-66) over the batch: predictor, residuals! at the
+// predicted fields, corrector, and for PECE residuals! at the corrected fields.  The step
+// moves along the residual that the previous step left (its own last residuals! pass).
+struct EulerStepper : BatchStepper {
+    int method;   // 0 PEC, 1 PECE
+    double lam;
+    EulerStepper(int m, double l) : method(m), lam(l) {}
+    const char* noncanonical_error() const override {
+        return "batched SCFT runs systems in the canonical ensemble only";
+    }
+    bool starts_with_residuals() const override { return false; }
+    void setup(Batch& b, const std::vector<int>& run) override {
+        b.free_momentum();
+        b.mom_state.assign(b.sys.size(), Batch::MomState{});
+        for (int i : run) {
+            System& s = *b.sys[i];
+            b.mom_state[i].tmp = (double*)dev_alloc(sizeof(double) * s.n * s.nmon());
+        }
+    }
+    void update(Batch& b, const std::vector<int>& act, std::vector<BatchResult>&) override {
+        size_t Nmax = 0;
+        for (int i : act) Nmax = std::max(Nmax, b.sys[i]->n * b.sys[i]->nmon());
+        const double half = 0.5 * lam;
+        b.bind_momentum(act);
+        launch_stage_b(b.d_mom, (int)act.size(), STAGE_EULER_PRED, lam, half, 0.0, Nmax,
+                       b.stream);
+        b.residuals(act);
+        launch_stage_b(b.d_mom, (int)act.size(), STAGE_EULER_CORR, lam, half, 0.0, Nmax,
+                       b.stream);
+        if (method == 1) b.residuals(act);
     }
 };
 
@@ -546,6 +579,7 @@ void Batch::free_momentum() {
         dev_free(m.mt);
         dev_free(m.vt);
         dev_free(m.step);
+        dev_free(m.tmp);
     }
     mom_state.clear();
     dev_free(d_mom);
@@ -560,13 +594,37 @@ std::vector<BatchResult> Batch::scft_anderson(int nhist, int iter_euler, double 
     return scft_run(u, nsteps, ftol, mask, dom, stol);
 }
 
-std::vector<BatchResult> Batch::scft_momentum(int method, double lam, double /*gamma: NAG only*/,
-                                              double b1, double b2, int nsteps, double ftol,
+void Batch::bind_momentum(const std::vector<int>& act) {
+    if (act == mom_act && d_mom) return;
+    std::vector<MomTask> t;
+    for (int i : act) {
+        System& s = *sys[i];
+        const MomState& m = mom_state[i];
+        t.push_back({m.mt, m.vt, m.step, s.w, s.res, s.n * s.nmon(), m.tmp});
+    }
+    PF_HIP(hipStreamSynchronize(stream));   // queued work may read the old table
+    dev_free(d_mom);
+    d_mom = (MomTask*)dev_alloc(sizeof(MomTask) * t.size());
+    PF_HIP(hipMemcpyAsync(d_mom, t.data(), sizeof(MomTask) * t.size(), hipMemcpyHostToDevice,
+                          stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    mom_act = act;
+}
+
+std::vector<BatchResult> Batch::scft_momentum(int method, double lam, double gamma, double b1,
+                                              double b2, int nsteps, double ftol,
                                               const int* mask, const Domain* dom, double stol) {
-    if (method < 0 || method > 2)
-        throw Error(-1, "batched MomentumUpdater: method must be MOM, RMS or ADAM (NAG "
-                        "evaluates its residuals at a look-ahead point)");
-    MomentumStepper u(method, lam, b1, b2);
+    if (method < 0 || method > 3)
+        throw Error(-1, "batched MomentumUpdater: method must be MOM, RMS, ADAM or NAG");
+    MomentumStepper u(method, lam, gamma, b1, b2);
+    return scft_run(u, nsteps, ftol, mask, dom, stol);
+}
+
+std::vector<BatchResult> Batch::scft_euler(int method, double lam, int nsteps, double ftol,
+                                           const int* mask, const Domain* dom, double stol) {
+    if (method < 0 || method > 1)
+        throw Error(-1, "batched EulerUpdater: method must be PEC or PECE");
+    EulerStepper u(method, lam);
     return scft_run(u, nsteps, ftol, mask, dom, stol);
 }
 
@@ -631,8 +689,9 @@ std::vector<BatchResult> Batch::scft_run(BatchStepper& u, int nsteps, double fto
     // update, which is then its final state (scft! ends with one residuals! after the
     // last step)
     std::vector<char> fresh(ns, 0);
+    const bool lead = u.starts_with_residuals();
     while (!act.empty() && step < nsteps) {
-        if (queued)
+        if (queued || !lead)
             bind_active(act);
         else
             residuals(act);
@@ -649,7 +708,7 @@ std::vector<BatchResult> Batch::scft_run(BatchStepper& u, int nsteps, double fto
         // between iterations.  Each system's residuals depend on its own fields only: for
         // the systems that stay active they are what the next iteration computes, and for
         // a system that the test drops they are the final residuals! of scft!.
-        if (step + 1 < nsteps && !vcell) {
+        if (step + 1 < nsteps && !vcell && lead) {
             residuals(act);
             queued = true;
             for (int i : act) fresh[i] = 1;

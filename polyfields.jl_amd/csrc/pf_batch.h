@@ -51,14 +51,20 @@ struct StepTask {
     int h, euler;
 };
 
-// Momentum family state of one system (device arrays of N = n * nmon doubles)
+// Momentum family and Euler state of one system (device arrays of N = n * nmon doubles;
+// tmp: NAG's saved fields, the Euler corrector's fields)
 struct MomTask {
     double *mt, *vt, *step, *w;
     const double* r;
     size_t N;
+    double* tmp;
 };
 void launch_momentum_b(const MomTask* d, int nsys, int method, double lam, double b1, double b2,
                        size_t Nmax, hipStream_t st);
+// the pointwise stages of the updaters whose step evaluates residuals! inside it
+enum { STAGE_NAG_LOOK = 0, STAGE_NAG_STEP = 1, STAGE_EULER_PRED = 2, STAGE_EULER_CORR = 3 };
+void launch_stage_b(const MomTask* d, int nsys, int op, double lam, double b1, double b2,
+                    size_t Nmax, hipStream_t st);
 
 template <typename T>
 void launch_lw_fill_b(const LwTask<T>* lw, int nlw, const FillTask<T>* fill, int nfill,
@@ -129,12 +135,14 @@ class HostPool {
 };
 
 class Batch;
-// A field updater run over a batch by Batch::scft_run: every step starts with residuals!
-// of the active systems (done by the loop, possibly queued early), then update() moves
-// the fields of every active system.
+// A field updater run over a batch by Batch::scft_run.  For an updater whose step starts
+// with residuals! (Anderson, MOM, RMS, ADAM) the loop runs that pass (possibly queued
+// early), then update() moves the fields of every active system; NAG and the Euler
+// predictor-corrector evaluate their residuals inside update().
 struct BatchStepper {
     virtual ~BatchStepper() = default;
     virtual const char* noncanonical_error() const = 0;
+    virtual bool starts_with_residuals() const { return true; }
     // setup! after the first residuals! of the systems in `run` (src/simulate.jl:86)
     virtual void setup(Batch& b, const std::vector<int>& run) = 0;
     // the rest of step! for the systems in `act`; may mark systems diverged in res
@@ -202,17 +210,23 @@ class Batch {
                                            int nsteps, double ftol, const int* mask,
                                            const Domain* dom = nullptr, double stol = 0.0);
     // scft!(sys, MomentumUpdater(method, lam, gamma, beta1, beta2)) for method 0 MOM,
-    // 1 RMS, 2 ADAM (NAG evaluates its residuals at a look-ahead point, not batched)
+    // 1 RMS, 2 ADAM, 3 NAG
     std::vector<BatchResult> scft_momentum(int method, double lam, double gamma, double b1,
                                            double b2, int nsteps, double ftol, const int* mask,
                                            const Domain* dom = nullptr, double stol = 0.0);
+    // scft!(sys, EulerUpdater(method, lam)) for method 0 PEC, 1 PECE
+    std::vector<BatchResult> scft_euler(int method, double lam, int nsteps, double ftol,
+                                        const int* mask, const Domain* dom = nullptr,
+                                        double stol = 0.0);
     // the scft! loop of src/simulate.jl:60-148 over the batch with updater u
     std::vector<BatchResult> scft_run(BatchStepper& u, int nsteps, double ftol, const int* mask,
                                       const Domain* dom, double stol);
     // momentum state, per system (empty unless a momentum run set it up)
     struct MomState {
-        double *mt = nullptr, *vt = nullptr, *step = nullptr;
+        double *mt = nullptr, *vt = nullptr, *step = nullptr, *tmp = nullptr;
     };
+    // the MomTask table of the active set `act` on the device (rebuilt when the set changes)
+    void bind_momentum(const std::vector<int>& act);
     std::vector<MomState> mom_state;
     MomTask* d_mom = nullptr;
     std::vector<int> mom_act;        // active set d_mom was built for

@@ -372,6 +372,45 @@ void launch_momentum_b(const MomTask* d, int nsys, int method, double lam, doubl
     PF_HIP(hipGetLastError());
 }
 
+// The pointwise parts of NAG (src/updaters/momentum.jl:141-165) and of the Euler
+// predictor-corrector (src/updaters/euler.jl:36-66) for every system (blockIdx.y), each a
+// fusion of the one-system copies and axpys with the same sums per point:
+//   NAG_LOOK    tmp = w; w += lam mt                      (the look-ahead fields)
+//   NAG_STEP    momentum update with b1 = gamma; w = tmp; w += lam step
+//   EULER_PRED  w += lam r; tmp = w; tmp += b1 r          (b1 = lam / 2)
+//   EULER_CORR  tmp += b1 r; w = tmp
+__global__ void k_stage_b(const MomTask* __restrict__ t, int op, double lam, double b1,
+                          double b2) {
+    const MomTask m = t[blockIdx.y];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m.N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        if (op == STAGE_NAG_LOOK) {
+            const double wi = m.w[i];
+            m.tmp[i] = wi;
+            m.w[i] = wi + lam * m.mt[i];
+        } else if (op == STAGE_NAG_STEP) {
+            momentum_point(0, m.mt, m.vt, m.step, m.r, b1, b2, i);
+            m.w[i] = m.tmp[i] + lam * m.step[i];
+        } else if (op == STAGE_EULER_PRED) {
+            const double ri = m.r[i];
+            const double wp = m.w[i] + lam * ri;
+            m.w[i] = wp;
+            m.tmp[i] = wp + b1 * ri;
+        } else {
+            const double v = m.tmp[i] + b1 * m.r[i];
+            m.tmp[i] = v;
+            m.w[i] = v;
+        }
+    }
+}
+
+void launch_stage_b(const MomTask* d, int nsys, int op, double lam, double b1, double b2,
+                    size_t Nmax, hipStream_t st) {
+    const int g = (int)std::min<size_t>((Nmax + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_stage_b, dim3(g, nsys), dim3(256), 0, st, d, op, lam, b1, b2);
+    PF_HIP(hipGetLastError());
+}
+
 void launch_mix_b(const SysTask* d, const StepTask* s, int nsys, size_t Nmax, hipStream_t st) {
     const int g = (int)std::min<size_t>((Nmax + 255) / 256, 1024);
     hipLaunchKernelGGL(k_mix_b, dim3(g, nsys), dim3(256), 0, st, d, s);

@@ -302,9 +302,9 @@ end
 `scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler); domain, opts)` for every
 system of the batch in lockstep (`pf_batch_scft`, or `pf_batch_scft_domain` with a
 DomainUpdater, whose settings each system's own cell updater takes).  The second form
-takes the settings of an AndersonUpdater or a MomentumUpdater (:MOM, :RMS, :ADAM;
-`pf_batch_scft_momentum`).  A diverged system is reported with ferr = NaN instead of
-raising.
+takes the settings of an AndersonUpdater, a MomentumUpdater (:MOM, :RMS, :ADAM, :NAG;
+`pf_batch_scft_momentum`) or an EulerUpdater (:PEC, :PECE; `pf_batch_scft_euler`).  A
+diverged system is reported with ferr = NaN instead of raising.
 """
 function scft!(b::HIPBatch, u::HIPUpdater; domain = nothing,
                opts::SCFTOptions = SCFTOptions(), mask = nothing)
@@ -312,10 +312,17 @@ function scft!(b::HIPBatch, u::HIPUpdater; domain = nothing,
     u.kind === :anderson && return scft!(b, p.nhist; iter_euler = p.iter_euler,
                                          lam_euler = p.lam_euler, domain = domain, opts = opts,
                                          mask = mask)
-    u.kind === :momentum || throw(ArgumentError("batches run AndersonUpdater or MomentumUpdater"))
     r = Vector{PFResults}(undef, length(b.systems))
     m = mask === nothing ? C_NULL : Cint.(mask)
     dh = domain === nothing ? C_NULL : domain.handle
+    if u.kind === :euler
+        check(ccall((:pf_batch_scft_euler, LIB), Cint,
+                    (Ptr{Cvoid}, Cint, Float64, Ptr{Cvoid}, Cint, Float64, Float64, Ptr{Cint},
+                     Ptr{PFResults}),
+                    b.handle, p.method === :PECE ? 1 : 0, p.lam, dh, opts.nsteps, opts.ftol,
+                    opts.stol, m, r))
+        return [SCFTResults(x.time, x.nsteps, x.converged != 0, x.ferr, x.serr, x.fhelm) for x in r]
+    end
     check(ccall((:pf_batch_scft_momentum, LIB), Cint,
                 (Ptr{Cvoid}, Cint, Float64, Float64, Float64, Float64, Ptr{Cvoid}, Cint, Float64,
                  Float64, Ptr{Cint}, Ptr{PFResults}),

@@ -94,6 +94,7 @@ SIGNATURES = {
     "pf_batch_scft": (I, [P, I, I, D, I, D, IP, C.POINTER(ScftResults)]),
     "pf_batch_scft_domain": (I, [P, I, I, D, P, I, D, D, IP, C.POINTER(ScftResults)]),
     "pf_batch_scft_momentum": (I, [P, I, D, D, D, D, P, I, D, D, IP, C.POINTER(ScftResults)]),
+    "pf_batch_scft_euler": (I, [P, I, D, P, I, D, D, IP, C.POINTER(ScftResults)]),
     "pf_batch_timing": (I, [P, I]),
     "pf_batch_set_graph": (I, [P, I]),
     "pf_system_autotune": (I, [P, I, IP, DP]),

@@ -530,8 +530,9 @@ class EulerUpdater(_Updater):
     def __init__(self, method="PECE", lam=0.05):
         if method not in ("PEC", "PECE"):
             raise ValueError("Invalid `method` parameter for EulerUpdater. Options include: (:PEC, :PECE)")
+        self.method, self.method_id, self.lam = method, 1 if method == "PECE" else 0, float(lam)
         h = C.c_void_p()
-        call("pf_updater_euler", C.byref(h), 1 if method == "PECE" else 0, float(lam))
+        call("pf_updater_euler", C.byref(h), self.method_id, self.lam)
         self._h = h
 
 
@@ -721,8 +722,8 @@ class Batch:
         system, or for those with a true ``mask`` entry; returns one SCFTResults per system
         (ferr NaN: diverged, or not run).  ``domain``: a DomainUpdater whose settings each
         system's own variable-cell updater takes (the object itself is not set up), or None
-        for a fixed cell; ``stol`` as in SCFTOptions.  ``updater``: an AndersonUpdater or
-        a MomentumUpdater (MOM, RMS or ADAM) whose settings every system's updater takes,
+        for a fixed cell; ``stol`` as in SCFTOptions.  ``updater``: an AndersonUpdater, a
+        MomentumUpdater or an EulerUpdater whose settings every system's updater takes,
         in place of the Anderson arguments."""
         res = (_CR * len(self.systems))()
         m = None
@@ -738,8 +739,14 @@ class Batch:
                  float(stol), iptr(m) if m is not None else None, res)
             return [SCFTResults(r.time, r.nsteps, bool(r.converged), r.ferr, r.serr, r.fhelm)
                     for r in res]
+        elif isinstance(updater, EulerUpdater):
+            call("pf_batch_scft_euler", self._h, updater.method_id, updater.lam,
+                 domain._h if domain is not None else None, int(nsteps), float(ftol),
+                 float(stol), iptr(m) if m is not None else None, res)
+            return [SCFTResults(r.time, r.nsteps, bool(r.converged), r.ferr, r.serr, r.fhelm)
+                    for r in res]
         elif updater is not None:
-            raise TypeError("batches run AndersonUpdater or MomentumUpdater")
+            raise TypeError("batches run AndersonUpdater, MomentumUpdater or EulerUpdater")
         if domain is not None:
             call("pf_batch_scft_domain", self._h, int(nhist), int(iter_euler),
                  float(lam_euler), domain._h, int(nsteps), float(ftol), float(stol),

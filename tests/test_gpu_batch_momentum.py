@@ -1,6 +1,7 @@
-"""Batched SCFT with the momentum updaters (pf_batch_scft_momentum): every system follows
-scft!(sys, MomentumUpdater(; method, lam, ...); domain) as the one-at-a-time path runs it
-(src/updaters/momentum.jl:47-139, src/simulate.jl:60-148).
+"""Batched SCFT with the momentum and Euler updaters (pf_batch_scft_momentum,
+pf_batch_scft_euler): every system follows scft!(sys, updater; domain) as the
+one-at-a-time path runs it (src/updaters/momentum.jl:47-165, src/updaters/euler.jl:36-66,
+src/simulate.jl:60-148).
 
 Fixed cell: 2-D hexagonal melts of different grids with :MOM, :RMS and :ADAM.  Variable
 cell: the 64^3 workload's composition and seed protocol on a 16^3 grid (tests/config2.py:
@@ -14,6 +15,7 @@ import numpy as np
 import pytest
 
 from config2 import A_CELL, PROTO, build, seeded_fields
+from helpers import diblock_system
 
 P = pytest.importorskip("polyfields_hip")
 pytestmark = pytest.mark.gpu
@@ -92,10 +94,64 @@ def test_batch_momentum_variable_cell_matches_one_at_a_time_fp64():
         assert a != pytest.approx(A_CELL, rel=1e-3)   # the cell moved
 
 
-def test_batch_refuses_nag():
-    s = _hex(*HEX[0], seed=1)
-    with pytest.raises(P.PolyFieldsError, match="NAG"):
-        P.Batch([s]).scft(updater=P.MomentumUpdater(method="NAG", lam=0.1), nsteps=5)
+@pytest.mark.parametrize("make_updater", [
+    lambda: P.MomentumUpdater(method="NAG", lam=0.1, gamma=0.9),
+    lambda: P.EulerUpdater(method="PEC", lam=0.05),
+    lambda: P.EulerUpdater(method="PECE", lam=0.05)], ids=["NAG", "PEC", "PECE"])
+def test_batch_lookahead_updaters_match_one_at_a_time_fp64(make_updater):
+    # NAG (momentum.jl:141-165) and the Euler predictor-corrector (euler.jl:36-66) run
+    # residuals! inside their step: at NAG's look-ahead fields, at the Euler predictor
+    # (and, for PECE, at the corrected fields); the step after it moves along the
+    # residual the previous step left
+    def make():
+        return [_hex(*c, seed=400 + k) for k, c in enumerate(HEX)]
+
+    single = make()
+    ref = [P.scft(s, make_updater(), opts=P.SCFTOptions(nsteps=80, ftol=1e-5)) for s in single]
+    batch_sys = make()
+    res = P.Batch(batch_sys).scft(updater=make_updater(), nsteps=80, ftol=1e-5)
+    for k, (r, q) in enumerate(zip(res, ref)):
+        print(f"case {k}: batch {r.nsteps} {r.converged} {r.fhelm:.14g} {r.ferr:.6g}; "
+              f"single {q.nsteps} {q.converged} {q.fhelm:.14g} {q.ferr:.6g}", flush=True)
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
+        assert r.ferr == pytest.approx(q.ferr, rel=1e-9)
+        for m in (1, 2):
+            np.testing.assert_allclose(np.asarray(batch_sys[k].fields[m]),
+                                       np.asarray(single[k].fields[m]), rtol=0, atol=1e-10)
+
+
+def test_batch_euler_variable_cell_matches_one_at_a_time_fp64():
+    # EulerUpdater(:PECE) with a DomainUpdater on two lamellar melts: the cell update
+    # ends with residuals! at the new cell, which the next predictor moves along
+    def make():
+        out = []
+        for L, chi, seed in ((10.0, 0.2, 12345), (12.0, 0.25, 7)):
+            s = diblock_system(P, (64, 1, 1), 1, "lamellar", (L,), N=100, f=0.5, chi=chi,
+                               Ns=100, precision="fp64")[0]
+            P.fieldinit(s, seed=seed, scale=0.1)
+            out.append(s)
+        return out
+
+    dom = lambda: P.DomainUpdater(nper=10, nskip=5, lam=0.1)
+    single = make()
+    ref = []
+    for s in single:
+        r = P.scft(s, P.EulerUpdater(method="PECE", lam=0.1), domain=dom(),
+                   opts=P.SCFTOptions(nsteps=150, ftol=1e-4, stol=1e-4))
+        ref.append((r, float(s.cell.params[0])))
+    batch_sys = make()
+    res = P.Batch(batch_sys).scft(updater=P.EulerUpdater(method="PECE", lam=0.1),
+                                  domain=dom(), nsteps=150, ftol=1e-4, stol=1e-4)
+    for k, (r, (q, qa)) in enumerate(zip(res, ref)):
+        a = float(batch_sys[k].cell.params[0])
+        print(f"case {k}: batch {r.nsteps} {r.converged} fhelm {r.fhelm:.12g} L {a:.10g}; "
+              f"single {q.nsteps} {q.converged} fhelm {q.fhelm:.12g} L {qa:.10g}", flush=True)
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-8)
+        assert a == pytest.approx(qa, rel=1e-6)
 
 
 def _blend(L, chi_ab, seed):
