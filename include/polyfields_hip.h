@@ -243,8 +243,10 @@ int pf_scft_save(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_opti
                  pf_scft_results* res, pf_save_fn save, void* user);
 
 /* Batches.  Systems of one precision on one device advance together: every propagator
-   launch covers the lanes of all systems (grouped by grid size), every system-level
-   kernel covers all systems.  The systems stay owned by the caller and must outlive the
+   launch covers the lanes of all systems (grouped by grid size and propagator method),
+   every system-level kernel covers all systems.  A checkpointed system
+   (pf_system_set_checkpoint) runs its own segment-wise density! inside the batch's
+   launch sequence, which then runs eagerly instead of from a graph.  The systems stay owned by the caller and must outlive the
    batch; no other call may use them while a batch call runs.  pf_batch_scft runs
    scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler)) with a fixed cell for every
    system in lockstep, dropping each from the active set once ferr < ftol; `results`

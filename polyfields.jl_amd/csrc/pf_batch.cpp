@@ -227,7 +227,7 @@ void Batch::run(int kind, const std::vector<int>& act) {
         if (kind == 1)
             launch_residuals_b(d_act, (int)act.size(), max_parts(tasks, false), nmax, stream);
     };
-    if (!use_graph || eng->timing_on()) {
+    if (!use_graph || eng->timing_on() || !eng->capturable()) {
         seq();
         return;
     }

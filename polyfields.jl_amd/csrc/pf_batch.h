@@ -98,6 +98,9 @@ class BatchEngineBase {
     virtual long generation() const = 0;   // bumped by every rebuild
     virtual void invalidate() = 0;          // force a rebuild (schedule change)
     virtual bool timing_on() const = 0;
+    // false when the prepared active set holds a checkpointed system, whose density!
+    // synchronises with the host between segments: the launch sequence then runs eagerly
+    virtual bool capturable() const { return true; }
     // HIP-event timing of the propagator launches; kind: 0 slab, 1 pencil, 2 planar
     virtual void timing(int on) = 0;
     virtual double timing_ms(int kind, int* count) = 0;

@@ -1119,8 +1119,14 @@ class BatchEngine : public BatchEngineBase {
         fillt.clear();
         points.clear();
         nmax = 0;
+        ck.clear();
         for (int i : act) {
             System& s = *B->sys[i];
+            if (s.ckpt) {
+                // checkpointed stacks: the system's own segment-wise density! (below)
+                ck.push_back(i);
+                continue;
+            }
             Engine<T>& e = E(i);
             e.prepare();
             nmax = std::max(nmax, s.n);
@@ -1231,8 +1237,12 @@ class BatchEngine : public BatchEngineBase {
     }
 
    public:
+    std::vector<int> ck;             // checkpointed systems of the prepared active set
+    hipEvent_t ck_before = nullptr, ck_after = nullptr;
     explicit BatchEngine(Batch& b) : B(&b) {
         PF_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        PF_HIP(hipEventCreateWithFlags(&ck_before, hipEventDisableTiming));
+        PF_HIP(hipEventCreateWithFlags(&ck_after, hipEventDisableTiming));
     }
     ~BatchEngine() override {
         dev_free(d_lanes);
@@ -1245,6 +1255,8 @@ class BatchEngine : public BatchEngineBase {
             (void)hipEventDestroy(se.second);
         }
         (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(ck_before);
+        (void)hipEventDestroy(ck_after);
         for (auto e : evpool) (void)hipEventDestroy(e);
     }
 
@@ -1297,9 +1309,25 @@ class BatchEngine : public BatchEngineBase {
     long generation() const override { return gen; }
     void invalidate() override { cached.clear(); }
     bool timing_on() const override { return tim; }
+    bool capturable() const override { return ck.empty(); }
+
+    // density! of the checkpointed systems of the active set, each through its own engine
+    // on its own stream (segments recomputed from their checkpoints, src/species/
+    // multiblock.jl:125-231 in that order), ordered after the batch's earlier work and
+    // before its later launches
+    void launch_ckpt() {
+        for (int i : ck) {
+            System& s = *B->sys[i];
+            PF_HIP(hipEventRecord(ck_before, B->stream));
+            PF_HIP(hipStreamWaitEvent(s.stream, ck_before, 0));
+            E(i).density(s);
+            PF_HIP(hipEventRecord(ck_after, s.stream));
+            PF_HIP(hipStreamWaitEvent(B->stream, ck_after, 0));
+        }
+    }
 
     void launch(const std::vector<int>& act) override {
-        launch_sweeps(act);
+        run_sweeps();
         hipStream_t st = B->stream;
         const int nparts = reduce_parts(nmax);
         if (!tasks.empty())
@@ -1307,10 +1335,18 @@ class BatchEngine : public BatchEngineBase {
         for (const PointTask& p : points)
             launch_point_density(p.w, p.vol, p.n, p.rho_sp, p.partial, p.nparts, p.sc, p.isp, st);
         if (!tasks.empty()) launch_simpson<T>(d_tasks, (int)tasks.size(), nmax, st);
+        launch_ckpt();
     }
 
+    // a checkpointed system keeps no full stacks to sweep into: it runs its whole density!
     void launch_sweeps(const std::vector<int>&) override {
+        run_sweeps();
+        launch_ckpt();
+    }
+
+    void run_sweeps() {
         hipStream_t st = B->stream;
+        if (lwt.empty() && fillt.empty()) return;   // checkpointed systems only
         launch_lw_fill_b<T>(d_lwt, (int)lwt.size(), d_fillt, (int)fillt.size(), nmax, st);
         PF_HIP(hipEventRecord(fork, st));
         hipEvent_t prev = fork;

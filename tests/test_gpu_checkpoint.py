@@ -121,3 +121,69 @@ def test_checkpointed_variable_cell_fp64():
         out.append((r.fhelm, c.params[0]))
     assert out[1][0] == pytest.approx(out[0][0], rel=3e-7)
     assert out[1][1] == pytest.approx(out[0][1], rel=3e-6)
+
+
+def test_checkpointed_systems_in_a_batch_fp64():
+    # checkpointed systems join a batch: each runs its own segment-wise density! inside the
+    # batch's launch sequence (eagerly, not from a graph), beside systems with full stacks;
+    # densities equal one-at-a-time bit for bit, and a batched Anderson scft! (fixed cell)
+    # and a variable-cell one match one-at-a-time
+    def make():
+        out = []
+        for k, (name, build) in enumerate(CASES):
+            s = build()
+            if k != 1:
+                s.set_checkpoint(-1 if k % 2 == 0 else 3)
+            rng = np.random.default_rng(40 + k)
+            for m in s.mids:
+                s.fields[m] = 0.3 * rng.standard_normal(s.shape)
+            out.append(s)
+        return out
+
+    single, batch_sys = make(), make()
+    for s in single:
+        s.density_update()
+    b = P.Batch(batch_sys)
+    b.density_update()
+    for a, c in zip(single, batch_sys):
+        for m in a.mids:
+            np.testing.assert_array_equal(np.asarray(c.density[m]), np.asarray(a.density[m]))
+    ref = [P.scft(s, P.AndersonUpdater(nhist=10), opts=P.SCFTOptions(nsteps=60, ftol=1e-7))
+           for s in single]
+    res = b.scft(nhist=10, nsteps=60, ftol=1e-7)
+    for k, (r, q) in enumerate(zip(res, ref)):
+        print(f"{CASES[k][0]}: batch {r.nsteps} {r.converged} {r.fhelm:.14g}; single {q.nsteps} "
+              f"{q.converged} {q.fhelm:.14g}", flush=True)
+        assert r.nsteps == q.nsteps and r.converged == q.converged
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
+
+
+def test_checkpointed_variable_cell_in_a_batch_fp64():
+    # a checkpointed lamellar melt and a full-stack one with a DomainUpdater in one batch:
+    # the stress passes recompute the checkpointed segments, as one at a time
+    def make():
+        out = []
+        for K, L in ((-1, 9.0), (0, 11.0)):
+            s = diblock_system(P, (64, 1, 1), 1, "lamellar", (L,), N=100, f=0.5, chi=0.2,
+                               Ns=100, precision="fp64")[0]
+            s.set_checkpoint(K)
+            P.fieldinit(s, seed=12345, scale=0.1)
+            out.append(s)
+        return out
+
+    dom = lambda: P.DomainUpdater(nper=10, nskip=5, lam=0.1)
+    single = make()
+    ref = []
+    for s in single:
+        r = P.scft(s, P.AndersonUpdater(nhist=10), domain=dom(),
+                   opts=P.SCFTOptions(nsteps=120, ftol=1e-4, stol=1e-4))
+        ref.append((r, float(s.cell.params[0])))
+    batch_sys = make()
+    res = P.Batch(batch_sys).scft(nhist=10, domain=dom(), nsteps=120, ftol=1e-4, stol=1e-4)
+    for k, (r, (q, qa)) in enumerate(zip(res, ref)):
+        a = float(batch_sys[k].cell.params[0])
+        print(f"case {k}: batch {r.nsteps} {r.converged} fhelm {r.fhelm:.12g} L {a:.10g}; "
+              f"single {q.nsteps} {q.converged} fhelm {q.fhelm:.12g} L {qa:.10g}", flush=True)
+        assert r.nsteps == q.nsteps and r.converged == q.converged
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-8)
+        assert a == pytest.approx(qa, rel=1e-6)
