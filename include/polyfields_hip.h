@@ -321,7 +321,9 @@ int pf_batch_scft_domain(pf_batch* b, int nhist, int iter_euler, double lam_eule
 /* The batched loop with scft!(sys, MomentumUpdater(; method, lam, gamma, beta1, beta2);
    domain) for every system (src/updaters/momentum.jl:47-165): method 0 :MOM, 1 :RMS,
    2 :ADAM, 3 :NAG (residuals! of every active system at its look-ahead fields, inside
-   the step).  domain NULL: a fixed cell (stol unused); else as pf_batch_scft_domain.
+   the step).  Systems may be in the grand ensemble (Anderson's loops refuse them, as
+   updaters/anderson.jl:61-63 does).  domain NULL: a fixed cell (stol unused); else as
+   pf_batch_scft_domain.
    One kernel applies every active system's moment update and field step.  No
    counterpart in the reference beyond scft! itself. */
 int pf_batch_scft_momentum(pf_batch* b, int method, double lam, double gamma, double beta1,
@@ -330,7 +332,7 @@ int pf_batch_scft_momentum(pf_batch* b, int method, double lam, double gamma, do
 /* The batched loop with scft!(sys, EulerUpdater(; method, lam); domain) for every system
    (src/updaters/euler.jl:36-66): method 0 :PEC, 1 :PECE.  Every step runs the predictor
    of all active systems in one kernel, one batched residuals! at the predicted fields,
-   the corrector, and for PECE a second batched residuals!.  domain as in
+   the corrector, and for PECE a second batched residuals!.  Systems and domain as in
    pf_batch_scft_momentum.  No counterpart in the reference beyond scft! itself. */
 int pf_batch_scft_euler(pf_batch* b, int method, double lam, const pf_domain* domain,
                         int nsteps, double ftol, double stol, const int* mask,

@@ -284,7 +284,10 @@ static double ferr_of(const double* v) {
     return std::sqrt(v[0]) / std::sqrt(2.220446049250313e-16 + v[1]);
 }
 
-static double fhelm_of(const System& s, const double* v) {
+// free_energy (src/system/system.jl:402-433) from the batch's five system sums; in the
+// grand ensemble the species fractions come from the device (muphi! of the last pass)
+static double fhelm_of(System& s, const double* v) {
+    if (!s.canonical) s.fetch_scalars();
     double ftrans = 0.0;
     for (size_t i = 0; i < s.species.size(); ++i)
         if (s.phi[i] > 1e-8) ftrans += s.phi[i] * (s.phi[i] - 1.0) / s.species[i].Nref;
@@ -407,9 +410,7 @@ struct MomentumStepper : BatchStepper {
     double lam, gamma, b1, b2;
     MomentumStepper(int m, double l, double g, double bb1, double bb2)
         : method(m), lam(l), gamma(g), b1(bb1), b2(bb2) {}
-    const char* noncanonical_error() const override {
-        return "batched SCFT runs systems in the canonical ensemble only";
-    }
+    bool allows_grand() const override { return true; }
     // NAG evaluates residuals! at its look-ahead fields inside the step
     bool starts_with_residuals() const override { return method != 3; }
     void setup(Batch& b, const std::vector<int>& run) override {
@@ -454,9 +455,7 @@ struct EulerStepper : BatchStepper {
     int method;   // 0 PEC, 1 PECE
     double lam;
     EulerStepper(int m, double l) : method(m), lam(l) {}
-    const char* noncanonical_error() const override {
-        return "batched SCFT runs systems in the canonical ensemble only";
-    }
+    bool allows_grand() const override { return true; }
     bool starts_with_residuals() const override { return false; }
     void setup(Batch& b, const std::vector<int>& run) override {
         b.free_momentum();
@@ -640,9 +639,11 @@ std::vector<BatchResult> Batch::scft_run(BatchStepper& u, int nsteps, double fto
     const bool vcell = dom != nullptr;
     if (vcell && dom->nskip < 1) throw Error(-1, "DomainUpdater nskip must be at least 1");
     if (act.empty()) return res;
+    bool grand = false;
     for (int i : act) {
         System* s = sys[i];
-        if (!s->canonical) throw Error(-2, u.noncanonical_error());
+        if (!s->canonical && !u.allows_grand()) throw Error(-2, u.noncanonical_error());
+        grand = grand || !s->canonical;
         s->validate();
         s->sync();
     }
@@ -707,8 +708,10 @@ std::vector<BatchResult> Batch::scft_run(BatchStepper& u, int nsteps, double fto
         // the convergence test reads its sums, so the device does not wait for the host
         // between iterations.  Each system's residuals depend on its own fields only: for
         // the systems that stay active they are what the next iteration computes, and for
-        // a system that the test drops they are the final residuals! of scft!.
-        if (step + 1 < nsteps && !vcell && lead) {
+        // a system that the test drops they are the final residuals! of scft!.  Not with a
+        // grand-ensemble system, whose free energy reads species fractions that the queued
+        // pass would overwrite.
+        if (step + 1 < nsteps && !vcell && lead && !grand) {
             residuals(act);
             queued = true;
             for (int i : act) fresh[i] = 1;

@@ -144,7 +144,10 @@ class Batch;
 // predictor-corrector evaluate their residuals inside update().
 struct BatchStepper {
     virtual ~BatchStepper() = default;
-    virtual const char* noncanonical_error() const = 0;
+    // the grand ensemble is allowed (the reference refuses it for Anderson mixing only),
+    // or the error a grand-ensemble system raises
+    virtual bool allows_grand() const { return false; }
+    virtual const char* noncanonical_error() const { return ""; }
     virtual bool starts_with_residuals() const { return true; }
     // setup! after the first residuals! of the systems in `run` (src/simulate.jl:86)
     virtual void setup(Batch& b, const std::vector<int>& run) = 0;

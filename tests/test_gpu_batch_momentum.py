@@ -246,3 +246,51 @@ def test_batch_rqm4_matches_one_at_a_time_fp64():
               flush=True)
         assert r.nsteps == q.nsteps and r.converged == q.converged
         assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
+
+
+def _grand(mu, seed):
+    # diblock + homopolymer + point species at fixed chemical potentials (grand ensemble)
+    cell = P.UnitCell(2, "rectangular", 6.0, 4.5)
+    a, b, c = P.Monomer(1), P.Monomer(2), P.Monomer(3)
+    s = P.FieldSystem((16, 12, 1), cell, monomers=[a, b, c], ensemble="grand",
+                      precision="fp64")
+    for sp, v in zip((P.Diblock(a, b, 40, 0.5, 1.0, 1.0, 20), P.Homopolymer(a, 20, 1.0, 10),
+                      P.Point(c)), mu):
+        s.add_species(sp, v)
+    fl = P.FloryInteraction()
+    fl.set_interaction(1, 2, 0.4)
+    fl.set_interaction(1, 3, 0.2)
+    s.add_interaction(fl)
+    rng = np.random.default_rng(seed)
+    for m in (1, 2, 3):
+        s.fields[m] = 0.1 * rng.standard_normal((1, 12, 16))
+    return s
+
+
+@pytest.mark.parametrize("make_updater", [
+    lambda: P.MomentumUpdater(method="MOM", lam=0.05),
+    lambda: P.MomentumUpdater(method="NAG", lam=0.05, gamma=0.9),
+    lambda: P.EulerUpdater(method="PECE", lam=0.05)], ids=["MOM", "NAG", "PECE"])
+def test_batch_grand_ensemble_matches_one_at_a_time_fp64(make_updater):
+    # grand-ensemble systems in a batch (momentum and Euler updaters; the reference refuses
+    # the grand ensemble for Anderson mixing only, anderson.jl:61-63): the free energy
+    # takes each system's species fractions phi = Q e^mu from its last pass
+    cases = [((0.1, -0.3, -1.2), 5), ((0.3, -0.5, -1.0), 6), ((-0.2, -0.1, -1.4), 7)]
+    single = [_grand(*c) for c in cases]
+    ref = [P.scft(s, make_updater(), opts=P.SCFTOptions(nsteps=60, ftol=1e-6)) for s in single]
+    batch_sys = [_grand(*c) for c in cases]
+    res = P.Batch(batch_sys).scft(updater=make_updater(), nsteps=60, ftol=1e-6)
+    for k, (r, q) in enumerate(zip(res, ref)):
+        print(f"grand {k}: batch {r.nsteps} {r.converged} {r.fhelm:.14g}; single {q.nsteps} "
+              f"{q.converged} {q.fhelm:.14g}", flush=True)
+        assert r.converged == q.converged
+        assert r.nsteps == q.nsteps
+        assert r.fhelm == pytest.approx(q.fhelm, rel=1e-12)
+        for m in (1, 2, 3):
+            np.testing.assert_allclose(np.asarray(batch_sys[k].density[m]),
+                                       np.asarray(single[k].density[m]), rtol=0, atol=1e-11)
+
+
+def test_batch_anderson_refuses_grand_ensemble():
+    with pytest.raises(P.PolyFieldsError, match="canonical"):
+        P.Batch([_grand((0.1, -0.3, -1.2), 5)]).scft(nhist=5, nsteps=3)
