@@ -85,6 +85,7 @@ class Engine : public EngineBase {
     int ntiles = 1, tile = 1;
     size_t nk = 0;
     int lds_plane = 0, lds_pencil = 0;
+    bool rq_one = false;             // RQM4 with one xy plane of LDS (two do not fit)
     // stress
     std::vector<StressLane<T>> slanes;
     StressLane<T>* d_slanes = nullptr;
@@ -177,8 +178,9 @@ class Engine : public EngineBase {
         tile = (nxh + ntiles - 1) / ntiles;
         lds_plane = (int)(sizeof(cpx<T>) * ny * nxh);
         lds_pencil = (int)(sizeof(cpx<T>) * nz * tile);
-        if (s.mde == 1 && 2 * lds_plane > 160 * 1024)
-            throw Error(-2, "RQM4 needs two xy planes in LDS; this plane is too large");
+        // RQM4 holds the full- and half-step planes in LDS together where two fit, else
+        // one after the other (lanes marked rqm4 = 2)
+        rq_one = s.mde == 1 && 2 * lds_plane > 160 * 1024;
         if (lds_plane > 160 * 1024)
             throw Error(-2, "xy plane of " + std::to_string(nx) + "x" + std::to_string(ny) +
                                 " does not fit the 160 KiB LDS of one workgroup");
@@ -225,7 +227,7 @@ class Engine : public EngineBase {
                 L.nsteps = sp.Ns;
                 L.nblk = sp.nblk();
                 for (int k = 0; k <= sp.nblk(); ++k) L.begin[k] = sp.begin[k];
-                L.rqm4 = s.mde == 1;
+                L.rqm4 = s.mde == 1 ? (rq_one ? 2 : 1) : 0;
                 for (int k = 0; k < sp.nblk(); ++k) {
                     L.lw[k] = b.lw[k];
                     if (s.mde == 1) L.lw2[k] = b.lw2[k];
@@ -456,6 +458,8 @@ class Engine : public EngineBase {
     int tiles() const { return ntiles; }
     int tile_w() const { return tile; }
     int plane_lds() const { return lds_plane; }
+    // LDS of an RQM4 slab or planar workgroup: one or two xy planes
+    int rqm4_lds() const { return rq_one ? lds_plane : 2 * lds_plane; }
     int pencil_lds() const { return lds_pencil; }
 
     explicit Engine(System& s) : S(&s) {}
@@ -524,7 +528,7 @@ class Engine : public EngineBase {
             if (planar) {
                 timed(2, [&] {
                     if (s.mde == 1)
-                        launch_planar_steps<T>(d_lanes, nl, 0, maxNs, 2 * lds_plane, 256, st);
+                        launch_planar_steps<T>(d_lanes, nl, 0, maxNs, rqm4_lds(), 256, st);
                     else if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
                         launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st);
                 });
@@ -532,13 +536,13 @@ class Engine : public EngineBase {
                 // RQM4: per contour step two spectra through the pencils and three slab
                 // launches (the next step's forward transforms ride on RQ_END)
                 const int nz = s.dims[2], ny = s.dims[1];
-                launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_START, 0, 2 * lds_plane, st);
+                launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_START, 0, rqm4_lds(), st);
                 for (int t = 0; t < maxNs; ++t) {
                     launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 0);
                     launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
-                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_MID, t, 2 * lds_plane, st);
+                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_MID, t, rqm4_lds(), st);
                     launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
-                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_END, t, 2 * lds_plane, st);
+                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_END, t, rqm4_lds(), st);
                 }
             } else {
                 const int nz = s.dims[2], ny = s.dims[1];
@@ -1032,6 +1036,7 @@ class BatchEngine : public BatchEngineBase {
     struct Group {
         int nx, ny, nz, off, count, maxNs, ntiles, tile, lds_plane, lds_pencil;
         int mde;   // 0 RK2, 1 RQM4 (lanes are grouped by grid size and method)
+        int rq_lds;   // LDS of an RQM4 slab or planar workgroup (one or two planes)
         hipStream_t st = nullptr;
         hipEvent_t done = nullptr;
     };
@@ -1144,6 +1149,7 @@ class BatchEngine : public BatchEngineBase {
                 ng.ntiles = e.tiles();
                 ng.tile = e.tile_w();
                 ng.lds_plane = e.plane_lds();
+                ng.rq_lds = e.rqm4_lds();
                 ng.lds_pencil = e.pencil_lds();
                 g.push_back(ng);
                 by_size.emplace_back();
@@ -1358,7 +1364,7 @@ class BatchEngine : public BatchEngineBase {
             if (gr.nz == 1 && gr.mde == 1) {
                 // RQM4 planes: the generic kernel with both planes in LDS (mde.jl:83-104)
                 timed(2, gr.st, gr.count, [&] {
-                    launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, 2 * gr.lds_plane, 256, gr.st);
+                    launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.rq_lds, 256, gr.st);
                 });
             } else if (gr.nz == 1) {
                 timed(2, gr.st, gr.count, [&] {
@@ -1368,13 +1374,13 @@ class BatchEngine : public BatchEngineBase {
             } else if (gr.mde == 1) {
                 // RQM4 in 3-D: per contour step the full- and half-step spectra through the
                 // pencils and three slab launches, as the one-system engine runs them
-                launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_START, 0, 2 * gr.lds_plane, gr.st);
+                launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_START, 0, gr.rq_lds, gr.st);
                 for (int t = 0; t < gr.maxNs; ++t) {
                     launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 0);
                     launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 1);
-                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_MID, t, 2 * gr.lds_plane, gr.st);
+                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_MID, t, gr.rq_lds, gr.st);
                     launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 1);
-                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_END, t, 2 * gr.lds_plane, gr.st);
+                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_END, t, gr.rq_lds, gr.st);
                 }
             } else {
                 timed(0, gr.st, gr.count, [&] {

@@ -38,6 +38,60 @@ __global__ void __launch_bounds__(256) k_planar_steps(const Lane<T>* __restrict_
             S[y * ld + (i - y * M)] = qin[i];
         }
     }
+    if (L->rqm4 == 2) {
+        // RQM4 with one plane of LDS (planes too large for two): the half steps re-read the
+        // step's input from the stack, and the full step's result q1 waits in the output
+        // slice until the combination.  Each thread reads back only the entries it wrote
+        // itself, and the products are those of the two-plane form below.
+        const int nkp = ny * (M + 1);
+        for (int t = t0; t < tend; ++t) {
+            const int b = step_block(L, t, &s_in);
+            const cpx<T>* lw = reinterpret_cast<const cpx<T>*>(L->lw[b]);
+            const cpx<T>* lw2 = reinterpret_cast<const cpx<T>*>(L->lw2[b]);
+            const cpx<T>* qin = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n);
+            cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n);
+            const double mc = -L->ldc[b];
+            for (int pass = 0; pass < 3; ++pass) {
+                const bool half = pass > 0;
+                if (pass < 2) {
+                    __syncthreads();
+                    for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                        const int y = i / M, o = y * ld + (i - y * M);
+                        const cpx<T> w = pass == 0 ? lw[i] : lw2[i];
+                        const cpx<T> v = pass == 0 ? S[o] : qin[i];
+                        S[o] = {w.x * v.x, w.y * v.y};
+                    }
+                    __syncthreads();
+                }
+                plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+                for (int i = threadIdx.x; i < nkp; i += blockDim.x) {
+                    const int y = i / (M + 1), kx = i - y * (M + 1);
+                    const double a = mc * ksq_at(L->G, (double)kx, (double)fold(y, ny), 0.0);
+                    const T f = exp_t<T>(half ? a / 2.0 : a);
+                    S[y * ld + kx].x *= f;
+                    S[y * ld + kx].y *= f;
+                }
+                __syncthreads();
+                plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+                const cpx<T>* wm = pass == 2 ? lw2 : lw;   // q1 *= LW1; mid *= LW1; q2 *= LW2
+                for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                    const int y = i / M, o = y * ld + (i - y * M);
+                    const cpx<T> w = wm[i];
+                    S[o] = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+                    if (pass == 0) qout[i] = S[o];
+                }
+                __syncthreads();
+            }
+            for (int i = threadIdx.x; i < np; i += blockDim.x) {
+                const int y = i / M, o = y * ld + (i - y * M);
+                const cpx<T> q1 = qout[i], q2 = S[o];
+                const cpx<T> v = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+                S[o] = v;
+                qout[i] = v;
+            }
+        }
+        return;
+    }
     if (L->rqm4) {
         // RQM4 (src/system/mde.jl:83-104): q1 = full step, q2 = two half steps with LW1 in
         // the middle, q_out = (4 q2 - q1) / 3.  S2 is a second plane of LDS.
@@ -204,7 +258,9 @@ __global__ void __launch_bounds__(256) k_slab(const Lane<T>* __restrict__ lanes,
 //   pencil:  LD2 on kb2
 //   RQ_END:  q_out <- (4 LW2 F^-1 kb2 - rq) / 3, then for the next step
 //            kbuf <- F LW1' q_out, kb2 <- F LW2' q_out
-// Two xy planes of LDS (S, S2).
+// Two xy planes of LDS (S, S2), or one (lanes with rqm4 == 2, planes too large for two):
+// the two transforms of a mode then run one after the other, re-reading the mode's input
+// (q_s, or q_out just written by the same threads) from global memory.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_slab_rqm4(const Lane<T>* __restrict__ lanes, int nz,
                                                    int t) {
@@ -252,6 +308,70 @@ __global__ void __launch_bounds__(256) k_slab_rqm4(const Lane<T>* __restrict__ l
         spec_out(kb, S);
         spec_out(kb2, S2);
     };
+    if (L->rqm4 == 2) {
+        // one plane of LDS: the first transform's spectrum is out before the second starts
+        // (src: the mode's input plane in global memory, multiplied by LW1 then LW2)
+        auto fwd_pair = [&](const cpx<T>* src, const cpx<T>* w1, const cpx<T>* w2) {
+            pairs([&](int i, int o) {
+                const cpx<T> v = src[i], w = w1[i];
+                S[o] = {v.x * w.x, v.y * w.y};
+            });
+            __syncthreads();
+            plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+            spec_out(kb, S);
+            __syncthreads();
+            pairs([&](int i, int o) {
+                const cpx<T> v = src[i], w = w2[i];
+                S[o] = {v.x * w.x, v.y * w.y};
+            });
+            __syncthreads();
+            plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+            spec_out(kb2, S);
+        };
+        if constexpr (MODE == RQ_START) {
+            const cpx<T>* q = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
+                                                              (size_t)z * plane);
+            fwd_pair(q, lw1, lw2);
+        } else if constexpr (MODE == RQ_MID) {
+            spec_in(S, kb);
+            __syncthreads();
+            plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+            pairs([&](int i, int o) {
+                const cpx<T> w = lw1[i], v = S[o];
+                rq[i] = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+            });
+            __syncthreads();
+            spec_in(S, kb2);
+            __syncthreads();
+            plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+            pairs([&](int i, int o) {
+                const cpx<T> w = lw1[i], m = S[o];
+                S[o] = {(m.x * scale) * w.x, (m.y * scale) * w.y};
+            });
+            __syncthreads();
+            plane_forward(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+            spec_out(kb2, S);
+        } else {
+            spec_in(S, kb2);
+            __syncthreads();
+            plane_inverse(S, M, ny, ld, L->px, L->py, L->twx, L->twy);
+            cpx<T>* qout = reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
+                                                     (size_t)z * plane);
+            pairs([&](int i, int o) {
+                const cpx<T> w = lw2[i], v = S[o], q1 = rq[i];
+                const cpx<T> q2 = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+                qout[i] = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+            });
+            if (t + 1 >= L->nsteps) return;
+            int s2;
+            const int b2 = step_block(L, t + 1, &s2);
+            const cpx<T>* lw1n = reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * plane);
+            const cpx<T>* lw2n = reinterpret_cast<const cpx<T>*>(L->lw2[b2] + (size_t)z * plane);
+            __syncthreads();
+            fwd_pair(qout, lw1n, lw2n);
+        }
+        return;
+    }
     if constexpr (MODE == RQ_START) {
         const cpx<T>* q = reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
                                                           (size_t)z * plane);

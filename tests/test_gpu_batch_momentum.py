@@ -211,13 +211,15 @@ def test_batch_sweeps_fill_the_stacks_as_density_does():
 
 
 def test_batch_rqm4_matches_one_at_a_time_fp64():
-    # RQM4 systems (mde.jl:83-104) in a batch beside an RK2 one: 1-D, 2-D and 3-D grids,
-    # the densities of one batched density! and an Anderson scft! equal one-at-a-time
+    # RQM4 systems (mde.jl:83-104) in a batch beside an RK2 one: 1-D, 2-D and 3-D grids and
+    # a 128 x 128 FP64 plane (one plane of LDS), the densities of one batched density! and
+    # an Anderson scft! equal one-at-a-time
     def make():
         out = []
         for k, (dims, dim, cr, prm) in enumerate([((64, 1, 1), 1, "lamellar", (12.0,)),
                                                   ((32, 32, 1), 2, "hexagonal", (8.0,)),
-                                                  ((16, 16, 16), 3, "cubic", (7.0,))]):
+                                                  ((16, 16, 16), 3, "cubic", (7.0,)),
+                                                  ((128, 128, 1), 2, "square", (12.0,))]):
             cell = P.UnitCell(dim, cr, *prm)
             A, B = P.Monomer(1), P.Monomer(2)
             ch = P.Multiblock([A, B], [35, 65], [1.0, 1.0], 40)

@@ -434,7 +434,12 @@ def test_batch_3d_density_fp32_matches_single():
                                                      ((24, 12, 1), 2, "rectangular", (7.0, 4.0)),
                                                      ((8, 8, 8), 3, "cubic", (6.0,)),
                                                      ((12, 8, 6), 3, "orthorhombic",
-                                                      (7.0, 5.0, 4.0))])
+                                                      (7.0, 5.0, 4.0)),
+                                                     # 128 x 128 FP64 planes: one plane of
+                                                     # LDS, the two sub-steps in turn
+                                                     ((128, 128, 1), 2, "square", (12.0,)),
+                                                     ((128, 128, 4), 3, "tetragonal",
+                                                      (12.0, 3.0))])
 def test_rqm4_propagator_fp64(dims, dim, crystal, params):
     def build(mod, precision=None):
         extra = {} if precision is None else {"precision": precision}
