@@ -198,9 +198,10 @@ int pf_density_vjp(pf_system* sys, const double* rho_bar, const double* Q_bar, d
    0: full stacks, the default).  Each chain then keeps q and q+ at the checkpoints plus one
    segment of K+1 slices per direction, about 2 (Ns/K + K) slices instead of 2 (Ns+1);
    density!, residuals!, free energies, pf_scfstress (so also the DomainUpdater),
-   pf_fhelm_grad and pf_density_vjp recompute segments as they need them.  The propagator
-   read-back and batches need full stacks and raise.  No counterpart in the reference, which
-   keeps every slice (species/multiblock.jl:109-110). */
+   pf_fhelm_grad and pf_density_vjp recompute segments as they need them, with RK2 or RQM4
+   steps (the gradient is RK2 only).  The propagator read-back needs full stacks and
+   raises.  No counterpart in the reference, which keeps every slice
+   (species/multiblock.jl:109-110). */
 int pf_system_set_checkpoint(pf_system* sys, int K);
 /* Set the block lengths of chain species isp to continuous values N_block[nblk]; every
    block keeps its number of contour steps and ds_b = v_b N_b / steps_b. */
@@ -246,8 +247,9 @@ int pf_scft_save(pf_system* sys, pf_updater* u, pf_domain* d, const pf_scft_opti
    launch covers the lanes of all systems (grouped by grid size and propagator method),
    every system-level kernel covers all systems.  A checkpointed system
    (pf_system_set_checkpoint) runs its own segment-wise density! inside the batch's
-   launch sequence, which then runs eagerly instead of from a graph.  The systems stay owned by the caller and must outlive the
-   batch; no other call may use them while a batch call runs.  pf_batch_scft runs
+   launch sequence, which then runs eagerly instead of from a graph.  The systems stay
+   owned by the caller and must outlive the batch; no other call may use them while a
+   batch call runs.  pf_batch_scft runs
    scft!(sys, AndersonUpdater(; nhist, iter_euler, lam_euler)) with a fixed cell for every
    system in lockstep, dropping each from the active set once ferr < ftol; `results`
    receives one entry per system (time is the whole batch's wall time).  A system whose

@@ -430,7 +430,6 @@ int pf_system_set_checkpoint(pf_system* sys, int K) {
     return guard([&] {
         auto& s = S(sys);
         need(K >= -1, "segment length");
-        if (K != 0 && s.mde != 0) throw pf::Error(-2, "checkpointed sweeps are provided for RK2 only");
         s.sync();
         s.ckpt = K;
         s.dirty_species = 1;

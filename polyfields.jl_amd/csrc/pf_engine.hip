@@ -197,7 +197,6 @@ class Engine : public EngineBase {
             b.row0 = row;
             row += sp.Ns + 1;
             if (ckK) {
-                if (s.mde != 0) throw Error(-2, "checkpointed sweeps are provided for RK2 only");
                 // segments of K slices between checkpoints; every chain gets a real q+ sweep
                 const int K = ckK > 0 ? ckK : (int)std::ceil(std::sqrt((double)sp.Ns + 1.0));
                 b.bnd.clear();
@@ -345,9 +344,22 @@ class Engine : public EngineBase {
         int t0, t1;
         seg_steps(j, m, &t0, &t1);
         const Lane<T>* dl = d_seglanes + segoff[j] + m;
-        if (planar) {
+        if (planar && s.mde == 1) {
+            launch_planar_steps<T>(dl, 1, t0, t1, rqm4_lds(), 256, st);
+        } else if (planar) {
             if (!fixed || !fixed_planar<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, st))
                 launch_planar_steps<T>(dl, 1, t0, t1, lds_plane, 256, st);
+        } else if (s.mde == 1) {
+            // RQM4 steps of the segment (the one-system launch sequence from step t0)
+            const int nz = s.dims[2], ny = s.dims[1];
+            launch_slab_rqm4<T>(dl, 1, nz, RQ_START, t0, rqm4_lds(), st);
+            for (int t = t0; t < t1; ++t) {
+                launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st, 0);
+                launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
+                launch_slab_rqm4<T>(dl, 1, nz, RQ_MID, t, rqm4_lds(), st);
+                launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
+                launch_slab_rqm4<T>(dl, 1, nz, RQ_END, t, rqm4_lds(), st);
+            }
         } else {
             const int nz = s.dims[2], ny = s.dims[1];
             if (!fixed || !fixed_slab<T>(dl, 1, s.dims[0], ny, nz, SLAB_FWD, t0, st))

@@ -187,3 +187,31 @@ def test_checkpointed_variable_cell_in_a_batch_fp64():
         assert r.nsteps == q.nsteps and r.converged == q.converged
         assert r.fhelm == pytest.approx(q.fhelm, rel=1e-8)
         assert a == pytest.approx(qa, rel=1e-6)
+
+
+@pytest.mark.parametrize("dims,dim,crystal,params", [((64, 1, 1), 1, "lamellar", (9.0,)),
+                                                     ((24, 16, 1), 2, "rectangular", (7.0, 5.0)),
+                                                     ((128, 128, 1), 2, "square", (12.0,)),
+                                                     ((16, 16, 16), 3, "cubic", (6.0,))])
+def test_checkpointed_rqm4_matches_full_fp64(dims, dim, crystal, params):
+    # RQM4 steps (mde.jl:83-104) recomputed segment by segment from the checkpoints give
+    # the full-stack density!, Q, free energy, residuals and stress (the 128^2 FP64 plane
+    # runs its sub-steps in one plane of LDS)
+    def build():
+        return diblock_system(P, dims, dim, crystal, params, N=100, f=0.35, chi=0.2, Ns=40,
+                              mde="RQM4", precision="fp64")[0]
+
+    full, ck = build(), build()
+    ck.set_checkpoint(-1)
+    rng = np.random.default_rng(33)
+    f = {m: 0.4 * rng.standard_normal(full.shape) for m in full.mids}
+    for s in (full, ck):
+        for m, w in f.items():
+            s.fields[m] = w
+        s.residuals_update()
+    assert ck.species[0].Q == pytest.approx(full.species[0].Q, rel=1e-12)
+    for m in full.mids:
+        assert rel(ck.density[m], full.density[m]) < 1e-12
+    assert ck.free_energy() == pytest.approx(full.free_energy(), rel=1e-12)
+    assert ck.field_error() == pytest.approx(full.field_error(), rel=1e-11)
+    assert rel(ck.scfstress(), full.scfstress()) < 1e-10
