@@ -345,7 +345,8 @@ class Engine : public EngineBase {
         seg_steps(j, m, &t0, &t1);
         const Lane<T>* dl = d_seglanes + segoff[j] + m;
         if (planar && s.mde == 1) {
-            launch_planar_steps<T>(dl, 1, t0, t1, rqm4_lds(), 256, st);
+            if (rq_one || !fixed || !fixed_planar_rqm4<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, st))
+                launch_planar_steps<T>(dl, 1, t0, t1, rqm4_lds(), 256, st);
         } else if (planar) {
             if (!fixed || !fixed_planar<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, st))
                 launch_planar_steps<T>(dl, 1, t0, t1, lds_plane, 256, st);
@@ -539,9 +540,11 @@ class Engine : public EngineBase {
         if (nl) {
             if (planar) {
                 timed(2, [&] {
-                    if (s.mde == 1)
-                        launch_planar_steps<T>(d_lanes, nl, 0, maxNs, rqm4_lds(), 256, st);
-                    else if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
+                    if (s.mde == 1) {
+                        if (rq_one || !fixed ||
+                            !fixed_planar_rqm4<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
+                            launch_planar_steps<T>(d_lanes, nl, 0, maxNs, rqm4_lds(), 256, st);
+                    } else if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
                         launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st);
                 });
             } else if (s.mde == 1) {
@@ -1374,9 +1377,12 @@ class BatchEngine : public BatchEngineBase {
             PF_HIP(hipStreamWaitEvent(gr.st, tim ? prev : fork, 0));
             const Lane<T>* dl = d_lanes + gr.off;
             if (gr.nz == 1 && gr.mde == 1) {
-                // RQM4 planes: the generic kernel with both planes in LDS (mde.jl:83-104)
+                // RQM4 planes (mde.jl:83-104): the compile-time kernel where both planes fit
+                // in LDS and the size is specialised, else the generic one
                 timed(2, gr.st, gr.count, [&] {
-                    launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.rq_lds, 256, gr.st);
+                    if (gr.rq_lds != 2 * gr.lds_plane ||
+                        !fixed_planar_rqm4<T>(dl, gr.count, gr.nx, gr.ny, 0, gr.maxNs, gr.st))
+                        launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.rq_lds, 256, gr.st);
                 });
             } else if (gr.nz == 1) {
                 timed(2, gr.st, gr.count, [&] {

@@ -114,6 +114,105 @@ __global__ void __launch_bounds__(NT) k_planar_fx(const Lane<T>* __restrict__ la
     }
 }
 
+// RQM4 steps (src/system/mde.jl:83-104) of 1-D / 2-D lanes at compile-time sizes: the
+// generic planar kernel's RQM4 branch (the same products in the same order: the full step
+// q1 in S, the two half steps in S2 with LW1 between them, q_out = (4 q2 - q1) / 3), with
+// fixed radix plans, LDS twiddles and one workgroup of NT threads per lane.  Where they
+// fit beside the two planes, the block's full- and half-step LD factors are computed into
+// LDS once per block (exp(a) and exp(a / 2), as the generic kernel forms them per pass).
+template <typename T, int NX, int NY, int NT>
+__global__ void __launch_bounds__(NT) k_planar_rq_fx(const Lane<T>* __restrict__ lanes, int t0,
+                                                     int t1) {
+    constexpr int M = NX / 2, LD = M + 1, N = NX * NY, NKP = NY * LD;
+    constexpr size_t BASE = 2 * (size_t)NKP * sizeof(cpx<T>) + (size_t)(NX + NY) * sizeof(cpx<T>);
+    static_assert(BASE <= 160u * 1024u, "both planes must fit in LDS");
+    constexpr bool FTAB = BASE + 2 * (size_t)NKP * sizeof(T) <= 160u * 1024u;
+    __shared__ cpx<T> S[NKP];
+    __shared__ cpx<T> S2[NKP];
+    __shared__ cpx<T> TWX[NX];
+    __shared__ cpx<T> TWY[NY];
+    __shared__ T FL[FTAB ? 2 * NKP : 1];
+    const Lane<T>* L = lanes + blockIdx.x;
+    const int tend = min(t1, L->nsteps);
+    if (t0 >= tend) return;
+    lds_copy_in<T, NX, NT>(TWX, L->twx);
+    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    const T scale = T(2.0 / (double)N);
+    int s_in;
+    step_block(L, t0, &s_in);
+    {
+        const auto qin = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * N));
+        for_pairs<T, NX, NY, NT>([&](int i, int o) { S[o] = qin[i]; });
+    }
+    const double* G = L->G;
+    int bf = -1;   // block whose factors FL holds
+    for (int t = t0; t < tend; ++t) {
+        const int b = step_block(L, t, &s_in);
+        const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b]));
+        const auto lw2 = gp(reinterpret_cast<const cpx<T>*>(L->lw2[b]));
+        const double mc = -L->ldc[b];
+        if constexpr (FTAB) {
+            if (b != bf) {
+                bf = b;
+#pragma unroll 1
+                for (int i0 = 0; i0 < NKP; i0 += NT) {
+                    const int i = i0 + threadIdx.x;
+                    if (i < NKP) {
+                        const int y = i / LD, kx = i % LD;
+                        const double a = mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0);
+                        FL[i] = exp_t<T>(a);
+                        FL[NKP + i] = exp_t<T>(a / 2.0);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> w = lw[i], w2 = lw2[i], v = S[o];
+            S2[o] = {w2.x * v.x, w2.y * v.y};
+            S[o] = {w.x * v.x, w.y * v.y};
+        });
+        __syncthreads();
+#pragma unroll 1
+        for (int pass = 0; pass < 3; ++pass) {
+            cpx<T>* X = pass == 0 ? S : S2;
+            const bool half = pass > 0;
+            cplane_forward<T, NX, NY, NT>(X, TWX, TWY);
+#pragma unroll
+            for (int i0 = 0; i0 < NKP; i0 += NT) {
+                const int i = i0 + threadIdx.x;
+                if (i < NKP) {
+                    T f;
+                    if constexpr (FTAB) {
+                        f = FL[(half ? NKP : 0) + i];
+                    } else {
+                        const int y = i / LD, kx = i % LD;
+                        const double a = mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0);
+                        f = exp_t<T>(half ? a / 2.0 : a);
+                    }
+                    X[i].x *= f;
+                    X[i].y *= f;
+                }
+            }
+            __syncthreads();
+            cplane_inverse<T, NX, NY, NT>(X, TWX, TWY);
+            const auto wm = pass == 2 ? lw2 : lw;   // q1 *= LW1; mid *= LW1; q2 *= LW2
+            for_pairs<T, NX, NY, NT>([&](int i, int o) {
+                const cpx<T> w = wm[i];
+                X[o] = {(X[o].x * scale) * w.x, (X[o].y * scale) * w.y};
+            });
+            __syncthreads();
+        }
+        const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * N));
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> q1 = S[o], q2 = S2[o];
+            const cpx<T> v = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+            S[o] = v;
+            qout[i] = v;
+        });
+    }
+}
+
 // forward xy transform of the plane in LDS and its half spectrum out to kb
 template <typename T, int NX, int NY, int NT, bool PPS, typename K>
 __device__ __forceinline__ void slab_forward_out(cpx<T>* S, cpx<T>* S2, const cpx<T>* TWX,
@@ -951,6 +1050,35 @@ bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipS
     return false;
 }
 
+// RQM4 planes and lines with both planes in LDS (the caller checks that two fit)
+template <typename T>
+bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st) {
+    if (schedule(sizeof(T), nx, ny, 1).planar == 0) return false;
+#define PF_RQ(X, Y, NT)                                                                    \
+    if (nx == X && ny == Y) {                                                              \
+        hipLaunchKernelGGL((k_planar_rq_fx<T, X, Y, NT>), dim3(nl), dim3(NT), 0, st, d, t0, t1); \
+        PF_HIP(hipGetLastError());                                                         \
+        return true;                                                                       \
+    }
+    PF_RQ(32, 1, 64)
+    PF_RQ(64, 1, 64)
+    PF_RQ(128, 1, 64)
+    PF_RQ(256, 1, 128)
+    PF_RQ(32, 32, 1024)
+    PF_RQ(48, 48, 1024)
+    PF_RQ(64, 64, 1024)
+    // FP64 96^2: 512 threads (the 1024-thread form spills 43 VGPRs); FP32 128^2 keeps
+    // 1024 (a few spills; the 512-thread form spills more)
+    if constexpr (sizeof(T) == 4) {
+        PF_RQ(96, 96, 1024)
+        PF_RQ(128, 128, 1024)
+    } else {
+        PF_RQ(96, 96, 512)
+    }
+#undef PF_RQ
+    return false;
+}
+
 template <typename T>
 bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                 hipStream_t st) {
@@ -1110,6 +1238,8 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, bool alone, 
 
 template bool fixed_planar<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
 template bool fixed_planar<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
+template bool fixed_planar_rqm4<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
+template bool fixed_planar_rqm4<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
 template bool fixed_slab<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t);
 template bool fixed_slab<double>(const Lane<double>*, int, int, int, int, int, int, hipStream_t);
 template bool fixed_pencil<float>(const Lane<float>*, int, int, int, int, bool, int, hipStream_t);

@@ -195,6 +195,8 @@ void schedule_reset();   // every grid back to the built-in defaults
 template <typename T>
 bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st);
 template <typename T>
+bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st);
+template <typename T>
 bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                 hipStream_t st);
 // alone: no other lane group's launches run beside this one (a single system's sweep)

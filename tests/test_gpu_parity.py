@@ -435,6 +435,10 @@ def test_batch_3d_density_fp32_matches_single():
                                                      ((8, 8, 8), 3, "cubic", (6.0,)),
                                                      ((12, 8, 6), 3, "orthorhombic",
                                                       (7.0, 5.0, 4.0)),
+                                                     # compile-time planes (1024 and 512
+                                                     # threads)
+                                                     ((64, 64, 1), 2, "hexagonal", (9.0,)),
+                                                     ((96, 96, 1), 2, "square", (10.0,)),
                                                      # 128 x 128 FP64 planes: one plane of
                                                      # LDS, the two sub-steps in turn
                                                      ((128, 128, 1), 2, "square", (12.0,)),
@@ -486,6 +490,23 @@ def test_grand_ensemble_parity_fp64():
         assert rel(sg.residuals[m], so.residuals[m]) < 1e-10
         assert rel(sg.fields[m], so.fields[m]) < 1e-14
     assert sg.free_energy() == pytest.approx(so.free_energy(), rel=1e-12)
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_rqm4_planar_fp32_error_vs_fp64(n):
+    # the compile-time RQM4 plane in FP32 against the same sweep in FP64 (128^2: FP32 runs
+    # both planes in LDS, FP64 one plane at a time)
+    f = rand_fields((1, n, n))
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s, _, c = diblock_system(P, (n, n, 1), 2, "square", (11.0,), Ns=40, mde="RQM4",
+                                 precision=prec)
+        P.fieldinit(s, fields=f)
+        s.density_update()
+        out[prec] = (c.Q, {m: s.density[m] for m in (1, 2)})
+    assert out["fp32"][0] == pytest.approx(out["fp64"][0], rel=1e-5)
+    for m in (1, 2):
+        assert rel(out["fp32"][1][m], out["fp64"][1][m]) < 1e-5
 
 
 def test_rqm4_3d_fp32_error_vs_fp64():
