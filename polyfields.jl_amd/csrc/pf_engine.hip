@@ -49,6 +49,31 @@ static std::vector<cpx<T>> twiddles(int n) {
     return t;
 }
 
+// 3-D RQM4 steps [t0, t1) of nl lanes (src/system/mde.jl:83-104): per step the pencils LD1
+// on the full-step spectrum and LD2 on the half-step one, the RQ_MID slab, a pencil LD2 and
+// the RQ_END slab (which starts the next step's spectra).  Compile-time forms where the
+// size is specialised (slabs only where both planes fit in LDS: `two`), generic otherwise.
+template <typename T>
+void rqm4_sweep(const Lane<T>* dl, int nl, int nx, int ny, int nz, int t0, int t1, bool two,
+                bool fixed, int ntiles, int tile, int lds_pencil, int rq_lds, hipStream_t st) {
+    auto slab = [&](int mode, int t) {
+        if (!two || !fixed || !fixed_slab_rqm4<T>(dl, nl, nx, ny, nz, mode, t, st))
+            launch_slab_rqm4<T>(dl, nl, nz, mode, t, rq_lds, st);
+    };
+    auto pencil = [&](int t, int half) {
+        if (!fixed || !fixed_pencil_rqm4<T>(dl, nl, nx, ny, nz, t, half, st))
+            launch_pencil<T>(dl, nl, ny, ntiles, tile, t, lds_pencil, 256, st, half);
+    };
+    slab(RQ_START, t0);
+    for (int t = t0; t < t1; ++t) {
+        pencil(t, 0);
+        pencil(t, 1);
+        slab(RQ_MID, t);
+        pencil(t, 1);
+        slab(RQ_END, t);
+    }
+}
+
 template <typename T>
 class Engine : public EngineBase {
     struct SpBuf {
@@ -345,22 +370,15 @@ class Engine : public EngineBase {
         seg_steps(j, m, &t0, &t1);
         const Lane<T>* dl = d_seglanes + segoff[j] + m;
         if (planar && s.mde == 1) {
-            if (rq_one || !fixed || !fixed_planar_rqm4<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, st))
+            if (!fixed || !fixed_planar_rqm4<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, rq_one, st))
                 launch_planar_steps<T>(dl, 1, t0, t1, rqm4_lds(), 256, st);
         } else if (planar) {
             if (!fixed || !fixed_planar<T>(dl, 1, s.dims[0], s.dims[1], t0, t1, st))
                 launch_planar_steps<T>(dl, 1, t0, t1, lds_plane, 256, st);
         } else if (s.mde == 1) {
             // RQM4 steps of the segment (the one-system launch sequence from step t0)
-            const int nz = s.dims[2], ny = s.dims[1];
-            launch_slab_rqm4<T>(dl, 1, nz, RQ_START, t0, rqm4_lds(), st);
-            for (int t = t0; t < t1; ++t) {
-                launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st, 0);
-                launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
-                launch_slab_rqm4<T>(dl, 1, nz, RQ_MID, t, rqm4_lds(), st);
-                launch_pencil<T>(dl, 1, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
-                launch_slab_rqm4<T>(dl, 1, nz, RQ_END, t, rqm4_lds(), st);
-            }
+            rqm4_sweep<T>(dl, 1, s.dims[0], s.dims[1], s.dims[2], t0, t1, !rq_one, fixed, ntiles,
+                          tile, lds_pencil, rqm4_lds(), st);
         } else {
             const int nz = s.dims[2], ny = s.dims[1];
             if (!fixed || !fixed_slab<T>(dl, 1, s.dims[0], ny, nz, SLAB_FWD, t0, st))
@@ -541,8 +559,8 @@ class Engine : public EngineBase {
             if (planar) {
                 timed(2, [&] {
                     if (s.mde == 1) {
-                        if (rq_one || !fixed ||
-                            !fixed_planar_rqm4<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
+                        if (!fixed || !fixed_planar_rqm4<T>(d_lanes, nl, s.dims[0], s.dims[1], 0,
+                                                            maxNs, rq_one, st))
                             launch_planar_steps<T>(d_lanes, nl, 0, maxNs, rqm4_lds(), 256, st);
                     } else if (!fixed || !fixed_planar<T>(d_lanes, nl, s.dims[0], s.dims[1], 0, maxNs, st))
                         launch_planar_steps<T>(d_lanes, nl, 0, maxNs, lds_plane, 256, st);
@@ -550,15 +568,8 @@ class Engine : public EngineBase {
             } else if (s.mde == 1) {
                 // RQM4: per contour step two spectra through the pencils and three slab
                 // launches (the next step's forward transforms ride on RQ_END)
-                const int nz = s.dims[2], ny = s.dims[1];
-                launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_START, 0, rqm4_lds(), st);
-                for (int t = 0; t < maxNs; ++t) {
-                    launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 0);
-                    launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
-                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_MID, t, rqm4_lds(), st);
-                    launch_pencil<T>(d_lanes, nl, ny, ntiles, tile, t, lds_pencil, 256, st, 1);
-                    launch_slab_rqm4<T>(d_lanes, nl, nz, RQ_END, t, rqm4_lds(), st);
-                }
+                rqm4_sweep<T>(d_lanes, nl, s.dims[0], s.dims[1], s.dims[2], 0, maxNs, !rq_one,
+                              fixed, ntiles, tile, lds_pencil, rqm4_lds(), st);
             } else {
                 const int nz = s.dims[2], ny = s.dims[1];
                 timed(0, [&] {
@@ -1380,8 +1391,8 @@ class BatchEngine : public BatchEngineBase {
                 // RQM4 planes (mde.jl:83-104): the compile-time kernel where both planes fit
                 // in LDS and the size is specialised, else the generic one
                 timed(2, gr.st, gr.count, [&] {
-                    if (gr.rq_lds != 2 * gr.lds_plane ||
-                        !fixed_planar_rqm4<T>(dl, gr.count, gr.nx, gr.ny, 0, gr.maxNs, gr.st))
+                    if (!fixed_planar_rqm4<T>(dl, gr.count, gr.nx, gr.ny, 0, gr.maxNs,
+                                              gr.rq_lds != 2 * gr.lds_plane, gr.st))
                         launch_planar_steps<T>(dl, gr.count, 0, gr.maxNs, gr.rq_lds, 256, gr.st);
                 });
             } else if (gr.nz == 1) {
@@ -1392,14 +1403,9 @@ class BatchEngine : public BatchEngineBase {
             } else if (gr.mde == 1) {
                 // RQM4 in 3-D: per contour step the full- and half-step spectra through the
                 // pencils and three slab launches, as the one-system engine runs them
-                launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_START, 0, gr.rq_lds, gr.st);
-                for (int t = 0; t < gr.maxNs; ++t) {
-                    launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 0);
-                    launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 1);
-                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_MID, t, gr.rq_lds, gr.st);
-                    launch_pencil<T>(dl, gr.count, gr.ny, gr.ntiles, gr.tile, t, gr.lds_pencil, 256, gr.st, 1);
-                    launch_slab_rqm4<T>(dl, gr.count, gr.nz, RQ_END, t, gr.rq_lds, gr.st);
-                }
+                rqm4_sweep<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, 0, gr.maxNs,
+                              gr.rq_lds == 2 * gr.lds_plane, true, gr.ntiles, gr.tile,
+                              gr.lds_pencil, gr.rq_lds, gr.st);
             } else {
                 timed(0, gr.st, gr.count, [&] {
                     if (!fixed_slab<T>(dl, gr.count, gr.nx, gr.ny, gr.nz, SLAB_FWD, 0, gr.st))

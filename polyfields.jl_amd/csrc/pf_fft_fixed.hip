@@ -213,6 +213,169 @@ __global__ void __launch_bounds__(NT) k_planar_rq_fx(const Lane<T>* __restrict__
     }
 }
 
+// RQM4 steps of 2-D lanes whose two planes do not fit in LDS together (FP64 128^2): the
+// generic planar kernel's one-plane branch at a compile-time size.  The half steps re-read
+// the step's input from the stack and the full step's result q1 waits in the output slice;
+// each thread reads back only the entries it wrote itself.
+template <typename T, int NX, int NY, int NT>
+__global__ void __launch_bounds__(NT) k_planar_rq1_fx(const Lane<T>* __restrict__ lanes, int t0,
+                                                      int t1) {
+    constexpr int M = NX / 2, LD = M + 1, N = NX * NY, NKP = NY * LD;
+    __shared__ cpx<T> S[NKP];
+    __shared__ cpx<T> TWX[NX];
+    __shared__ cpx<T> TWY[NY];
+    const Lane<T>* L = lanes + blockIdx.x;
+    const int tend = min(t1, L->nsteps);
+    if (t0 >= tend) return;
+    lds_copy_in<T, NX, NT>(TWX, L->twx);
+    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    const T scale = T(2.0 / (double)N);
+    int s_in;
+    step_block(L, t0, &s_in);
+    {
+        const auto qin = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * N));
+        for_pairs<T, NX, NY, NT>([&](int i, int o) { S[o] = qin[i]; });
+    }
+    const double* G = L->G;
+    for (int t = t0; t < tend; ++t) {
+        const int b = step_block(L, t, &s_in);
+        const auto lw = gp(reinterpret_cast<const cpx<T>*>(L->lw[b]));
+        const auto lw2 = gp(reinterpret_cast<const cpx<T>*>(L->lw2[b]));
+        const auto qin = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * N));
+        const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * N));
+        const double mc = -L->ldc[b];
+#pragma unroll 1
+        for (int pass = 0; pass < 3; ++pass) {
+            const bool half = pass > 0;
+            if (pass < 2) {
+                __syncthreads();
+                for_pairs<T, NX, NY, NT>([&](int i, int o) {
+                    const cpx<T> w = pass == 0 ? lw[i] : lw2[i];
+                    const cpx<T> v = pass == 0 ? S[o] : qin[i];
+                    S[o] = {w.x * v.x, w.y * v.y};
+                });
+                __syncthreads();
+            }
+            cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+#pragma unroll 1
+            for (int i0 = 0; i0 < NKP; i0 += NT) {
+                const int i = i0 + threadIdx.x;
+                if (i < NKP) {
+                    const int y = i / LD, kx = i % LD;
+                    const double a = mc * ksq_at(G, (double)kx, (double)fold(y, NY), 0.0);
+                    const T f = exp_t<T>(half ? a / 2.0 : a);
+                    S[i].x *= f;
+                    S[i].y *= f;
+                }
+            }
+            __syncthreads();
+            cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
+            const auto wm = pass == 2 ? lw2 : lw;   // q1 *= LW1; mid *= LW1; q2 *= LW2
+            for_pairs<T, NX, NY, NT>([&](int i, int o) {
+                const cpx<T> w = wm[i];
+                S[o] = {(S[o].x * scale) * w.x, (S[o].y * scale) * w.y};
+                if (pass == 0) qout[i] = S[o];
+            });
+            __syncthreads();
+        }
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> q1 = qout[i], q2 = S[o];
+            const cpx<T> v = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+            S[o] = v;
+            qout[i] = v;
+        });
+    }
+}
+
+// 3-D RQM4 slab passes (src/system/mde.jl:83-104) at compile-time plane sizes, both planes
+// in LDS: the modes, products and order of operations of the generic k_slab_rqm4, with
+// fixed radix plans and LDS twiddles.
+template <typename T, int NX, int NY, int MODE, int NT>
+__global__ void __launch_bounds__(NT) k_slab_rq_fx(const Lane<T>* __restrict__ lanes, int nz, int t) {
+    constexpr int M = NX / 2, PLANE = NX * NY, KPLANE = (M + 1) * NY;
+    __shared__ cpx<T> S[KPLANE];
+    __shared__ cpx<T> S2[KPLANE];
+    __shared__ cpx<T> TWX[NX];
+    __shared__ cpx<T> TWY[NY];
+    // grid (nz, lanes): plane z of lane blockIdx.y
+    const int z = blockIdx.x;
+    const Lane<T>* L = lanes + blockIdx.y;
+    if (t >= L->nsteps) return;
+    const size_t n = (size_t)PLANE * nz;
+    lds_copy_in<T, NX, NT>(TWX, L->twx);
+    lds_copy_in<T, NY, NT>(TWY, L->twy);
+    cpx<T>* const kbp = L->kbuf + (size_t)z * KPLANE;
+    cpx<T>* const kb2p = L->kb2 + (size_t)z * KPLANE;
+    const auto rq = gp(reinterpret_cast<cpx<T>*>(L->rq + (size_t)z * PLANE));
+    const T scale = T(2.0 / (double)n);
+    int s_in;
+    const int b = step_block(L, t, &s_in);
+    const auto lw1 = gp(reinterpret_cast<const cpx<T>*>(L->lw[b] + (size_t)z * PLANE));
+    const auto lw2 = gp(reinterpret_cast<const cpx<T>*>(L->lw2[b] + (size_t)z * PLANE));
+    auto spec_out = [&](cpx<T>* dst, const cpx<T>* src) {
+        const auto g = gp(dst);
+#pragma unroll
+        for (int i0 = 0; i0 < KPLANE; i0 += NT)
+            if (i0 + (int)threadIdx.x < KPLANE) g[i0 + threadIdx.x] = src[i0 + threadIdx.x];
+    };
+    auto both_forward = [&] {
+        __syncthreads();
+        cplane_forward<T, NX, NY, NT>(S, TWX, TWY);
+        cplane_forward<T, NX, NY, NT>(S2, TWX, TWY);
+        spec_out(kbp, S);
+        spec_out(kb2p, S2);
+    };
+    if constexpr (MODE == RQ_START) {
+        const auto q = gp(reinterpret_cast<const cpx<T>*>(L->stack + (size_t)(s_in - 1) * n +
+                                                          (size_t)z * PLANE));
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> v = q[i], w = lw1[i], w2 = lw2[i];
+            S[o] = {v.x * w.x, v.y * w.y};
+            S2[o] = {v.x * w2.x, v.y * w2.y};
+        });
+        both_forward();
+    } else if constexpr (MODE == RQ_MID) {
+        lds_copy_in<T, KPLANE, NT>(S, kbp);
+        lds_copy_in<T, KPLANE, NT>(S2, kb2p);
+        __syncthreads();
+        cplane_inverse<T, NX, NY, NT>(S, TWX, TWY);
+        cplane_inverse<T, NX, NY, NT>(S2, TWX, TWY);
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> w = lw1[i], v = S[o], m = S2[o];
+            rq[i] = cpx<T>{(v.x * scale) * w.x, (v.y * scale) * w.y};
+            S2[o] = {(m.x * scale) * w.x, (m.y * scale) * w.y};
+        });
+        __syncthreads();
+        cplane_forward<T, NX, NY, NT>(S2, TWX, TWY);
+        spec_out(kb2p, S2);
+    } else {
+        lds_copy_in<T, KPLANE, NT>(S2, kb2p);
+        __syncthreads();
+        cplane_inverse<T, NX, NY, NT>(S2, TWX, TWY);
+        const auto qout = gp(reinterpret_cast<cpx<T>*>(L->stack + (size_t)(s_in + L->dir - 1) * n +
+                                                       (size_t)z * PLANE));
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> w = lw2[i], v = S2[o], q1 = rq[i];
+            const cpx<T> q2 = {(v.x * scale) * w.x, (v.y * scale) * w.y};
+            const cpx<T> r = {(T(4) * q2.x - q1.x) / T(3), (T(4) * q2.y - q1.y) / T(3)};
+            qout[i] = r;
+            S[o] = r;
+        });
+        if (t + 1 >= L->nsteps) return;
+        int s2;
+        const int b2 = step_block(L, t + 1, &s2);
+        const auto lw1n = gp(reinterpret_cast<const cpx<T>*>(L->lw[b2] + (size_t)z * PLANE));
+        const auto lw2n = gp(reinterpret_cast<const cpx<T>*>(L->lw2[b2] + (size_t)z * PLANE));
+        __syncthreads();
+        for_pairs<T, NX, NY, NT>([&](int i, int o) {
+            const cpx<T> v = S[o], w = lw1n[i], w2 = lw2n[i];
+            S[o] = {v.x * w.x, v.y * w.y};
+            S2[o] = {v.x * w2.x, v.y * w2.y};
+        });
+        both_forward();
+    }
+}
+
 // forward xy transform of the plane in LDS and its half spectrum out to kb
 template <typename T, int NX, int NY, int NT, bool PPS, typename K>
 __device__ __forceinline__ void slab_forward_out(cpx<T>* S, cpx<T>* S2, const cpx<T>* TWX,
@@ -601,7 +764,8 @@ __global__ void __launch_bounds__(256, 4) k_slab_sx(const Lane<float>* __restric
 // z-pencil pass over KB consecutive ky rows and all NXH kx columns: in the half-spectrum
 // layout [kz][ky][kx] those KB*NXH values are contiguous for every kz, so each z row is
 // one coalesced run.
-template <typename T, int NZ, int NXH, int KB, int NT>
+// HALF (RQM4): the half steps' spectrum kb2 with the LD exponent halved
+template <typename T, int NZ, int NXH, int KB, int NT, bool HALF = false>
 __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ lanes, int ny, int t) {
     constexpr int C = KB * NXH;
     __shared__ cpx<T> P[NZ * C];
@@ -611,7 +775,7 @@ __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ la
     // rows too and returns before its first barrier.
     const Lane<T>* L = lanes + blockIdx.y;
     const int ky0 = blockIdx.x * KB;
-    unsigned long long kraw = __builtin_bit_cast(unsigned long long, L->kbuf),
+    unsigned long long kraw = __builtin_bit_cast(unsigned long long, HALF ? L->kb2 : L->kbuf),
                        tzr = __builtin_bit_cast(unsigned long long, L->twz);
     int nsteps = L->nsteps, s_in;
     int b = __builtin_amdgcn_readfirstlane(step_block(L, t, &s_in));
@@ -640,7 +804,7 @@ __global__ void __launch_bounds__(NT) k_pencil_fx(const Lane<T>* __restrict__ la
     if (t >= nsteps) return;
     __syncthreads();
     cfft<T, NZ, C, C, 1, NT, false, 1>(P, TWZ);
-    const double mc = -pick_blk(L->ldc, b);
+    const double mc = HALF ? -pick_blk(L->ldc, b) / 2.0 : -pick_blk(L->ldc, b);
     const double* G = L->G;
 #pragma unroll
     for (int i0 = 0; i0 < NZ * C; i0 += NT) {
@@ -1050,10 +1214,23 @@ bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipS
     return false;
 }
 
-// RQM4 planes and lines with both planes in LDS (the caller checks that two fit)
+// RQM4 planes and lines; one: the two planes do not fit in LDS together (the caller's
+// engine decides), else both planes in LDS
 template <typename T>
-bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st) {
+bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, bool one,
+                       hipStream_t st) {
     if (schedule(sizeof(T), nx, ny, 1).planar == 0) return false;
+    if (one) {
+        if constexpr (sizeof(T) == 8) {
+            if (nx == 128 && ny == 128) {
+                hipLaunchKernelGGL((k_planar_rq1_fx<T, 128, 128, 256>), dim3(nl), dim3(256), 0, st,
+                                   d, t0, t1);
+                PF_HIP(hipGetLastError());
+                return true;
+            }
+        }
+        return false;
+    }
 #define PF_RQ(X, Y, NT)                                                                    \
     if (nx == X && ny == Y) {                                                              \
         hipLaunchKernelGGL((k_planar_rq_fx<T, X, Y, NT>), dim3(nl), dim3(NT), 0, st, d, t0, t1); \
@@ -1076,6 +1253,55 @@ bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1,
         PF_RQ(96, 96, 512)
     }
 #undef PF_RQ
+    return false;
+}
+
+// 3-D RQM4 with both planes in LDS (the caller checks that two fit): the slab modes and
+// the pencils (half: the half steps' spectrum) at compile-time sizes
+template <typename T>
+bool fixed_slab_rqm4(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
+                     hipStream_t st) {
+    if (schedule(sizeof(T), nx, ny, nz).slab == 0) return false;
+#define PF_RS(X, Y, NT)                                                                    \
+    if (nx == X && ny == Y) {                                                              \
+        if (mode == RQ_START)                                                              \
+            hipLaunchKernelGGL((k_slab_rq_fx<T, X, Y, RQ_START, NT>), dim3(nz, nl), dim3(NT), 0, \
+                               st, d, nz, t);                                              \
+        else if (mode == RQ_MID)                                                           \
+            hipLaunchKernelGGL((k_slab_rq_fx<T, X, Y, RQ_MID, NT>), dim3(nz, nl), dim3(NT), 0, \
+                               st, d, nz, t);                                              \
+        else                                                                               \
+            hipLaunchKernelGGL((k_slab_rq_fx<T, X, Y, RQ_END, NT>), dim3(nz, nl), dim3(NT), 0, \
+                               st, d, nz, t);                                              \
+        PF_HIP(hipGetLastError());                                                         \
+        return true;                                                                       \
+    }
+    PF_RS(32, 32, 256)
+    PF_RS(64, 64, 512)
+    if constexpr (sizeof(T) == 4) { PF_RS(128, 128, 512) }
+#undef PF_RS
+    return false;
+}
+
+template <typename T>
+bool fixed_pencil_rqm4(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, int half,
+                       hipStream_t st) {
+    if (schedule(sizeof(T), nx, ny, nz).pencil == 0) return false;
+#define PF_RP(X, Z, KB, NT)                                                                \
+    if (nx == X && nz == Z && ny % KB == 0) {                                              \
+        if (half)                                                                          \
+            hipLaunchKernelGGL((k_pencil_fx<T, Z, X / 2 + 1, KB, NT, true>), dim3(ny / KB, nl), \
+                               dim3(NT), 0, st, d, ny, t);                                 \
+        else                                                                               \
+            hipLaunchKernelGGL((k_pencil_fx<T, Z, X / 2 + 1, KB, NT, false>), dim3(ny / KB, nl), \
+                               dim3(NT), 0, st, d, ny, t);                                 \
+        PF_HIP(hipGetLastError());                                                         \
+        return true;                                                                       \
+    }
+    PF_RP(32, 32, 4, 256)
+    PF_RP(64, 64, 1, 512)
+    PF_RP(128, 128, 1, 512)
+#undef PF_RP
     return false;
 }
 
@@ -1238,8 +1464,14 @@ bool fixed_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, bool alone, 
 
 template bool fixed_planar<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
 template bool fixed_planar<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
-template bool fixed_planar_rqm4<float>(const Lane<float>*, int, int, int, int, int, hipStream_t);
-template bool fixed_planar_rqm4<double>(const Lane<double>*, int, int, int, int, int, hipStream_t);
+template bool fixed_planar_rqm4<float>(const Lane<float>*, int, int, int, int, int, bool,
+                                       hipStream_t);
+template bool fixed_planar_rqm4<double>(const Lane<double>*, int, int, int, int, int, bool,
+                                        hipStream_t);
+template bool fixed_slab_rqm4<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t);
+template bool fixed_slab_rqm4<double>(const Lane<double>*, int, int, int, int, int, int, hipStream_t);
+template bool fixed_pencil_rqm4<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t);
+template bool fixed_pencil_rqm4<double>(const Lane<double>*, int, int, int, int, int, int, hipStream_t);
 template bool fixed_slab<float>(const Lane<float>*, int, int, int, int, int, int, hipStream_t);
 template bool fixed_slab<double>(const Lane<double>*, int, int, int, int, int, int, hipStream_t);
 template bool fixed_pencil<float>(const Lane<float>*, int, int, int, int, bool, int, hipStream_t);

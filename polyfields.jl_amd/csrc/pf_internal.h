@@ -195,7 +195,14 @@ void schedule_reset();   // every grid back to the built-in defaults
 template <typename T>
 bool fixed_planar(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st);
 template <typename T>
-bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, hipStream_t st);
+bool fixed_planar_rqm4(const Lane<T>* d, int nl, int nx, int ny, int t0, int t1, bool one,
+                       hipStream_t st);
+template <typename T>
+bool fixed_slab_rqm4(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
+                     hipStream_t st);
+template <typename T>
+bool fixed_pencil_rqm4(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, int half,
+                       hipStream_t st);
 template <typename T>
 bool fixed_slab(const Lane<T>* d, int nl, int nx, int ny, int nz, int mode, int t,
                 hipStream_t st);

@@ -219,6 +219,7 @@ def test_batch_rqm4_matches_one_at_a_time_fp64():
         for k, (dims, dim, cr, prm) in enumerate([((64, 1, 1), 1, "lamellar", (12.0,)),
                                                   ((32, 32, 1), 2, "hexagonal", (8.0,)),
                                                   ((16, 16, 16), 3, "cubic", (7.0,)),
+                                                  ((32, 32, 32), 3, "cubic", (8.0,)),
                                                   ((128, 128, 1), 2, "square", (12.0,))]):
             cell = P.UnitCell(dim, cr, *prm)
             A, B = P.Monomer(1), P.Monomer(2)

@@ -435,6 +435,9 @@ def test_batch_3d_density_fp32_matches_single():
                                                      ((8, 8, 8), 3, "cubic", (6.0,)),
                                                      ((12, 8, 6), 3, "orthorhombic",
                                                       (7.0, 5.0, 4.0)),
+                                                     # compile-time 3-D slabs and pencils
+                                                     ((32, 32, 32), 3, "cubic", (8.0,)),
+                                                     ((64, 64, 64), 3, "cubic", (12.0,)),
                                                      # compile-time planes (1024 and 512
                                                      # threads)
                                                      ((64, 64, 1), 2, "hexagonal", (9.0,)),
@@ -509,10 +512,11 @@ def test_rqm4_planar_fp32_error_vs_fp64(n):
         assert rel(out["fp32"][1][m], out["fp64"][1][m]) < 1e-5
 
 
-def test_rqm4_3d_fp32_error_vs_fp64():
-    # the 3-D RQM4 sweep in FP32 against the same sweep in FP64
-    dims = (16, 16, 16)
-    f = rand_fields((16, 16, 16))
+@pytest.mark.parametrize("g", [16, 64])
+def test_rqm4_3d_fp32_error_vs_fp64(g):
+    # the 3-D RQM4 sweep in FP32 against the same sweep in FP64 (64^3: compile-time forms)
+    dims = (g, g, g)
+    f = rand_fields((g, g, g))
     out = {}
     for prec in ("fp64", "fp32"):
         s, _, c = diblock_system(P, dims, 3, "cubic", (7.0,), Ns=30, mde="RQM4", precision=prec)
