@@ -463,6 +463,43 @@ def grad_workload(P, args):
             "density_ms": dt_d * 1e3, "gradient_over_density": dt_g / dt_d}
 
 
+def rqm4_workload(P, args):
+    """One system's density! with RQM4 steps (src/system/mde.jl:83-104) against RK2 on the
+    same system: 64^3 cubic, 64^2 and 128^2 square diblocks (N=100, fA=0.35, chiN=20,
+    Ns=102), fields 0.3 N(0,1), FP64 and FP32."""
+    cases = [((64, 64, 64), 3, "cubic", 16.0), ((64, 64, 1), 2, "square", 9.0),
+             ((128, 128, 1), 2, "square", 12.0)]
+    out = {}
+    for prec in ("fp64", "fp32"):
+        for dims, dim, crystal, a in cases:
+            row = {}
+            for mde in ("RK2", "RQM4"):
+                cell = P.UnitCell(dim, crystal, a)
+                A, B = P.Monomer(1), P.Monomer(2)
+                ch = P.Diblock(A, B, 100, FA, 1.0, 1.0, 100)
+                itx = P.FloryInteraction()
+                itx.set_interaction(1, 2, CHI)
+                s = P.FieldSystem(dims, cell, monomers=[A, B], precision=prec, mde=mde)
+                s.add_species(ch, 1.0)
+                s.add_interaction(itx)
+                rng = np.random.default_rng(args.seed)
+                for m in s.mids:
+                    s.fields[m] = 0.3 * rng.standard_normal(s.shape)
+                row[mde] = time_calls(s.density_update, s.synchronize, args.steps,
+                                      args.warmup) * 1e3
+                del s
+            row["rqm4_over_rk2"] = row["RQM4"] / row["RK2"]
+            out[f"{prec} {'x'.join(str(d) for d in dims if d > 1)}"] = row
+    v = out["fp64 64x64x64"]["RQM4"]
+    return {"metric": "64^3 RQM4 density! time (FP64)", "value": v, "unit": "ms",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": v,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded 0.3*N(0,1))",
+            "config": {"workload": "one diblock system (N=100, fA=0.35, chiN=20, Ns=102), "
+                                   "RQM4 and RK2 density! per step"},
+            "ms_per_density": out}
+
+
 # config-2 SCFT: MomentumUpdater(:MOM, lam = 0.1) + DomainUpdater(nskip = 100, nper = 10),
 # ftol = stol = 1e-5 (the reference's Anderson mixing diverges from this seed)
 SCFT64 = dict(lam=0.1, nskip=100, nper=10, ftol=1e-5, stol=1e-5, nsteps=5000)
@@ -712,11 +749,12 @@ def main():
                          "of polyfields_hip.schedule_set) instead of autotuning")
     ap.add_argument("--workload", default="density64",
                     choices=["density64", "blend128", "grad64", "scft64", "scft64_batch",
-                             "sweep_vcell", "copy"],
+                             "sweep_vcell", "rqm4", "copy"],
                     help="density64: the headline (64^3 batch); blend128 / grad64 / scft64: "
                          "the large blend, the free-energy gradient and the 64^3 "
-                         "variable-cell SCFT, one JSON line each; copy: a plain device "
-                         "copy of known size (counter calibration)")
+                         "variable-cell SCFT, one JSON line each; rqm4: one system's "
+                         "density! with RQM4 against RK2; copy: a plain device copy of "
+                         "known size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the SCFT backend runs on a single MI355X "
@@ -728,7 +766,7 @@ def main():
     if args.workload != "density64":
         fn = {"blend128": blend_workload, "grad64": grad_workload,
               "scft64": scft64_workload, "scft64_batch": scft64_batch_workload,
-              "sweep_vcell": sweep_vcell_workload,
+              "sweep_vcell": sweep_vcell_workload, "rqm4": rqm4_workload,
               "copy": copy_workload}[args.workload]
         print(json.dumps(fn(P, args)))
         return
