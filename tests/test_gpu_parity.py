@@ -442,9 +442,12 @@ def test_batch_3d_density_fp32_matches_single():
                                                      # threads)
                                                      ((64, 64, 1), 2, "hexagonal", (9.0,)),
                                                      ((96, 96, 1), 2, "square", (10.0,)),
-                                                     # 128 x 128 FP64 planes: one plane of
-                                                     # LDS, the two sub-steps in turn
+                                                     # FP64 planes too large for two in LDS:
+                                                     # one plane, the sub-steps in turn
+                                                     # (compile-time 128^2, generic 108x96)
                                                      ((128, 128, 1), 2, "square", (12.0,)),
+                                                     ((108, 96, 1), 2, "rectangular",
+                                                      (11.0, 10.0)),
                                                      ((128, 128, 4), 3, "tetragonal",
                                                       (12.0, 3.0))])
 def test_rqm4_propagator_fp64(dims, dim, crystal, params):
