@@ -491,10 +491,38 @@ class Engine : public EngineBase {
     int plane_lds() const { return lds_plane; }
     // LDS of an RQM4 slab or planar workgroup: one or two xy planes
     int rqm4_lds() const { return rq_one ? lds_plane : 2 * lds_plane; }
+
+    // The RK2 sweep of lanes [off, off + cnt) (device table dl) over `steps` contour steps
+    // on stream ss: the first xy-slab forward pass, then pencil and slab per step.
+    void rk2_sweep(System& s, const Lane<T>* dl, int off, int cnt, int steps, hipStream_t ss) {
+        const int nz = s.dims[2], ny = s.dims[1];
+        if (!fixed || !fixed_slab<T>(dl, cnt, s.dims[0], ny, nz, SLAB_FWD, 0, ss))
+            launch_slab<T>(dl, cnt, nz, SLAB_FWD, 0, lds_plane, 256, ss);
+        for (int t = 0; t < steps; ++t) {
+            const int nla = live_lanes(lanes, off, cnt, t);
+            if (!fixed || !fixed_pencil<T>(dl, nla, s.dims[0], ny, nz, true, t, ss))
+                launch_pencil<T>(dl, nla, ny, ntiles, tile, t, lds_pencil, 256, ss);
+            if (!fixed || !fixed_slab<T>(dl, nla, s.dims[0], ny, nz, SLAB_INVFWD, t, ss))
+                launch_slab<T>(dl, nla, nz, SLAB_INVFWD, t, lds_plane, 256, ss);
+        }
+    }
+    // one system's 3-D RK2 lanes as two stream groups: three or more lanes, unless the
+    // grid's schedule asks for one stream (split = 1)
+    bool split_lanes(int nl) const {
+        const int sp = schedule(sizeof(T), S->dims[0], S->dims[1], S->dims[2]).split;
+        return nl >= 3 && sp != 1;
+    }
+    hipStream_t sub[2] = {nullptr, nullptr};
+    hipEvent_t sub_done[2] = {nullptr, nullptr}, sub_fork = nullptr;
     int pencil_lds() const { return lds_pencil; }
 
     explicit Engine(System& s) : S(&s) {}
     ~Engine() override {
+        for (int p = 0; p < 2; ++p) {
+            if (sub[p]) (void)hipStreamDestroy(sub[p]);
+            if (sub_done[p]) (void)hipEventDestroy(sub_done[p]);
+        }
+        if (sub_fork) (void)hipEventDestroy(sub_fork);
         if (hrows) (void)hipHostFree(hrows);
         free_all();
         dev_free(tw);
@@ -570,6 +598,27 @@ class Engine : public EngineBase {
                 // launches (the next step's forward transforms ride on RQ_END)
                 rqm4_sweep<T>(d_lanes, nl, s.dims[0], s.dims[1], s.dims[2], 0, maxNs, !rq_one,
                               fixed, ntiles, tile, lds_pencil, rqm4_lds(), st);
+            } else if (!tim && split_lanes(nl)) {
+                // three or more lanes: two groups on streams of their own, so that one
+                // group's slab launches fill the CUs the other's launches leave idle
+                if (!sub[0]) {
+                    for (int p = 0; p < 2; ++p) {
+                        PF_HIP(hipStreamCreateWithFlags(&sub[p], hipStreamNonBlocking));
+                        PF_HIP(hipEventCreateWithFlags(&sub_done[p], hipEventDisableTiming));
+                    }
+                    PF_HIP(hipEventCreateWithFlags(&sub_fork, hipEventDisableTiming));
+                }
+                PF_HIP(hipEventRecord(sub_fork, st));
+                const int c0 = nl / 2;
+                for (int p = 0; p < 2; ++p) {
+                    const int off = p == 0 ? 0 : c0, cnt = p == 0 ? c0 : nl - c0;
+                    int steps = 0;
+                    for (int j = off; j < off + cnt; ++j) steps = std::max(steps, lanes[j].nsteps);
+                    PF_HIP(hipStreamWaitEvent(sub[p], sub_fork, 0));
+                    rk2_sweep(s, d_lanes + off, off, cnt, steps, sub[p]);
+                    PF_HIP(hipEventRecord(sub_done[p], sub[p]));
+                    PF_HIP(hipStreamWaitEvent(st, sub_done[p], 0));
+                }
             } else {
                 const int nz = s.dims[2], ny = s.dims[1];
                 timed(0, [&] {
