@@ -1141,7 +1141,7 @@ class BatchEngine : public BatchEngineBase {
     std::vector<long> cell_versions;
     hipEvent_t fork = nullptr;
     std::vector<std::pair<hipStream_t, hipEvent_t>> streams;   // one per (sub-)group
-    // 3-D groups of at least 4 lanes run as 2 sub-groups on 2 streams by default: measured
+    // 3-D groups of at least 3 lanes run as 2 sub-groups on 2 streams by default: measured
     // on MI355X (64^3 FP32, 32 systems) 2 streams give +20% over 1, 3 no further gain
     static constexpr int split = 2;
     size_t nmax = 0;
@@ -1274,7 +1274,11 @@ class BatchEngine : public BatchEngineBase {
             const int cnt = (int)by_size[k].size();
             const int sp = schedule(sizeof(T), g[k].nx, g[k].ny, g[k].nz).split;
             const int want = sp > 0 ? sp : split;
-            const int parts = (g[k].nz > 1) ? std::max(1, std::min(want, cnt / 2)) : 1;
+            // (a group of three lanes splits too: a 128^3 blend batched alone, FP64
+            // 18.48 -> 16.43 ms per density!; two lanes split measured slower)
+            const int parts = (g[k].nz > 1 && cnt >= 3)
+                                  ? std::max(1, std::min(want, std::max(2, cnt / 2)))
+                                  : 1;
             int done = 0;
             for (int p = 0; p < parts; ++p) {
                 const int c = (cnt - done) / (parts - p);
