@@ -209,3 +209,30 @@ def test_blend_128_checkpointed_gradient(big):
         kept += 2 * (nseg + 1) + 2 * (K + 1)
     print(f"128^3 blend: slices held {full} (full stacks) vs {kept} (checkpointed), "
           f"{full * 128 ** 3 * 8 / 2 ** 30:.2f} GiB vs {kept * 128 ** 3 * 8 / 2 ** 30:.2f} GiB FP64")
+
+
+def test_blend_128_variable_cell_scft_checkpointed():
+    """The config's variable-cell SCFT on the 128^3 blend (FP64): scft! with a
+    DomainUpdater relaxing the three orthorhombic edges, on sqrt(Ns)-checkpointed stacks
+    (the stress passes recompute the segments) against the same run on full stacks.  The
+    forward-difference cell Jacobian (dx = 1e-8) scales the rounding gap between the two
+    Simpson summation orders (1e-15) by about 1e8, hence the bounds."""
+    out = []
+    for K in (0, -1):
+        s = blend(P, (128, 128, 128), cell_params(), precision="fp64")
+        s.set_checkpoint(K)
+        seed_fields(s)
+        start = np.array(s.cell.params, dtype=float)
+        r = P.scft(s, P.MomentumUpdater(method="MOM", lam=0.05),
+                   domain=P.DomainUpdater(nskip=4, nper=2, lam=0.1),
+                   opts=P.SCFTOptions(nsteps=12, ftol=1e-9, stol=1e-9))
+        cell = np.array(s.cell.params, dtype=float)
+        print(f"K={K}: nsteps {r.nsteps} fhelm {r.fhelm:.12g} ferr {r.ferr:.4g} serr "
+              f"{r.serr:.4g} cell {cell}", flush=True)
+        out.append((r, cell, start))
+    (rf, cf, start), (rc, cc, _) = out
+    assert rf.nsteps == rc.nsteps == 12
+    assert np.max(np.abs(cf - start)) > 1e-6 * np.max(start)     # the cell moved
+    assert rc.fhelm == pytest.approx(rf.fhelm, rel=1e-7)
+    np.testing.assert_allclose(cc, cf, rtol=1e-6)
+    assert rc.ferr == pytest.approx(rf.ferr, rel=1e-4)
