@@ -595,8 +595,9 @@ int fixed_adj_pencil(const Lane<T>* d, int nl, int nx, int ny, int nz, int t, hi
     // 64-point lines in launches with at most one ky row per CU: half rows (twice the
     // workgroups and partial-sum rows; the sums then run in another order)
     if (nx == 64 && nz == 64 && nl * ny <= device_cus()) {
-        hipLaunchKernelGGL((k_adj_pencil_cs<T, 64, 33, 256>), dim3(2 * ny, nl), dim3(256), 0, st,
-                           d, ny, t);
+        // 512 threads (measured on MI355X, 64^3 FP64 gradient: 17.2 -> 15.0 us per launch)
+        hipLaunchKernelGGL((k_adj_pencil_cs<T, 64, 33, 512>), dim3(2 * ny, nl), dim3(512), 0,
+                           st, d, ny, t);
         PF_HIP(hipGetLastError());
         return 2 * ny;
     }
